@@ -1,0 +1,208 @@
+/*
+ * metada_amd.h -- C-ABI of the MI355X (gfx950) ensemble Kalman analysis library.
+ *
+ * A drop-in for the analysis step behind metada's algorithm interface:
+ *   EnKF<Tag>::Analyse   src/framework/algorithms/EnKF.hpp:139-256
+ *   ETKF<Tag>::Analyse   src/framework/algorithms/ETKF.hpp:100-178
+ *   LETKF<Tag>::Analyse  src/framework/algorithms/LETKF.hpp:63-243
+ * plus the ensemble statistics they call (Ensemble.hpp:105-151), the
+ * observation-space quantities (EnKF.hpp:168-184) and Metrics (Metrics.hpp).
+ *
+ * Conventions (shared with the reference):
+ *   - Ensembles are member-major: data[j*n + i] is member j at the linear
+ *     state index i = lev*nx*ny + y*nx + x (SimpleState.hpp:72-78,
+ *     IdentityObsOperator.hpp:652).  Members are updated IN PLACE, as the
+ *     reference algorithms do (EnKF.hpp:230-234, LETKF.hpp:240-242).
+ *   - Observation-space ensembles H(x_j) are member-major [k][p]: one vector
+ *     per member, as ObsOperator::apply returns it (EnKF.hpp:168-172).
+ *   - Observation error enters as the R diagonal, i.e. variances
+ *     (GridObservation::getCovariance, GridObservation.hpp:239-252).
+ *   - Every entry point returns MDA_OK (0) or a negative MDA_E_* code; no C++
+ *     exception crosses the ABI (the reference's exceptions become codes;
+ *     mda_last_error() gives the message).  The WRFDA bridge uses the same
+ *     convention (WRFDAObsOperator_c_api.h:33,43-71).
+ *   - Pointers flagged on_device are HIP device pointers on the context's
+ *     device; otherwise they are host pointers and the library copies.
+ *   - One context per host thread; each context owns one HIP stream and a
+ *     device workspace that is reused across calls.
+ */
+#ifndef METADA_AMD_H
+#define METADA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDA_OK 0
+#define MDA_E_ARG (-1)         /* invalid argument / shape */
+#define MDA_E_DEVICE (-2)      /* HIP runtime error */
+#define MDA_E_NOMEM (-3)       /* device allocation failed */
+#define MDA_E_NUMERIC (-4)     /* non-SPD matrix, NaN/Inf input */
+#define MDA_E_UNSUPPORTED (-5) /* combination not implemented */
+
+typedef struct mda_ctx mda_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the implicit single-threaded host state of the reference
+ * (ApplicationContext.hpp:84-89 initialises config/logging only). */
+int mda_ctx_create(int device, mda_ctx** out);
+int mda_ctx_destroy(mda_ctx* ctx);
+const char* mda_last_error(const mda_ctx* ctx);
+int mda_ctx_synchronize(mda_ctx* ctx);
+/* hipStream_t of the context, as void* (callers that interleave their own
+ * kernels order them against this stream). */
+void* mda_ctx_stream(mda_ctx* ctx);
+/* Device workspace currently held by the context, in bytes. */
+int64_t mda_ctx_workspace_bytes(const mda_ctx* ctx);
+
+/* ---- data descriptors ----------------------------------------------------- */
+typedef struct {
+  int32_t k;               /* ensemble size (Ensemble::Size, Ensemble.hpp:99) */
+  int64_t nx, ny, nlev;    /* n = nx*ny*nlev */
+  double* data;            /* [k][n] contiguous, or NULL when members is used */
+  double* const* members;  /* host only: k separate member vectors (reference layout) */
+  int32_t on_device;       /* data is a device pointer */
+} mda_ensemble;
+
+enum { MDA_COORD_GRID = 0, MDA_COORD_GEOGRAPHIC = 1, MDA_COORD_CARTESIAN = 2 };
+
+typedef struct {
+  int64_t p;
+  const double* value;     /* y_o (GridObservation::getObservationValues) */
+  const double* variance;  /* R diagonal, error^2; +inf marks an invalid obs */
+  const double* loc_a;     /* GRID: x | GEOGRAPHIC: latitude deg | CARTESIAN: x */
+  const double* loc_b;     /* GRID: y | GEOGRAPHIC: longitude deg | CARTESIAN: y */
+  const double* loc_c;     /* CARTESIAN: z (may be NULL = 0) */
+  int32_t on_device;
+} mda_obs;
+
+enum { MDA_HOP_CSR = 0, MDA_HOP_HX = 1 };
+
+/* The observation operator: either a linear stencil in CSR form over state
+ * linear indices (the form IdentityObsOperator's IDW4 and the trilinear
+ * operator both take), or a caller-computed H(x_j) for operators that only
+ * exist on the host (ObsOperator::apply, ObsOperator.hpp:255-259). */
+typedef struct {
+  int32_t kind;            /* MDA_HOP_CSR | MDA_HOP_HX */
+  const int64_t* row_ptr;  /* CSR: [p+1] */
+  const int64_t* col;      /* CSR: state linear indices */
+  const double* weight;    /* CSR: stencil weights */
+  int64_t nnz;
+  const double* hx;        /* HX: [k][p] member-major */
+  int32_t on_device;
+} mda_hop;
+
+/* Column (horizontal point) locations for localisation.  GRID with NULL
+ * arrays means column g = y*nx + x sits at (x, y) (SimpleGeometry.hpp:42-55). */
+typedef struct {
+  int32_t coord;           /* MDA_COORD_* ; distances as Location::distance_to */
+  const double* col_a;     /* [nx*ny]: x | latitude deg | x */
+  const double* col_b;     /* [nx*ny]: y | longitude deg | y */
+  const double* col_c;     /* [nx*ny]: CARTESIAN z (may be NULL) */
+  int32_t on_device;
+} mda_geometry;
+
+/* ---- ensemble statistics --------------------------------------------------- */
+/* Ensemble::RecomputeMean / RecomputePerturbations (Ensemble.hpp:105-151):
+ * mean = (sum_j x_j) * (1/k); pert_j = (x_j - mean) * scale.  Either output
+ * may be NULL.  Outputs live on the device when out_on_device. */
+int mda_ensemble_mean_perturbations(mda_ctx* ctx, const mda_ensemble* ens, double scale,
+                                    double* mean, double* pert, int32_t out_on_device);
+
+/* Observation-space mean, anomalies and innovation (EnKF.hpp:168-184,
+ * LETKF.hpp:209-211): hx = H x_j, ybar, Y' = hx - ybar ([k][p]), d = y_o - ybar.
+ * Any output may be NULL. */
+int mda_obs_innovations(mda_ctx* ctx, const mda_ensemble* ens, const mda_obs* obs,
+                        const mda_hop* hop, double* ybar, double* yp, double* d,
+                        int32_t out_on_device);
+
+/* ---- EnKF (EnKF.hpp:105-256) ------------------------------------------------- */
+enum { MDA_INFL_MULTIPLICATIVE = 0, MDA_INFL_ADDITIVE = 1, MDA_INFL_RELAXATION = 2 };
+
+typedef struct { /* EnKF::AnalysisResults, EnKF.hpp:84-96 */
+  double innovation_norm;
+  double analysis_increment_norm; /* never assigned by the reference: 0 */
+  double background_spread;
+  double analysis_spread;
+  double max_kalman_gain;
+  double min_kalman_gain;
+  double condition_number;        /* NaN when not computed (see DESIGN) */
+  int32_t ensemble_size;
+  int64_t observation_count;
+  int32_t inflation_method;
+  double inflation_factor;
+} mda_enkf_diag;
+
+/* Stochastic EnKF with perturbed observations.  Z is the N(0,1) draw matrix
+ * the reference takes from std::random_device (EnKF.hpp:346-356), injected
+ * here, [p][k] row-major in the reference fill order (obs, member). */
+int mda_enkf_analyse(mda_ctx* ctx, double inflation, int32_t inflation_method,
+                     mda_ensemble* ens, const mda_obs* obs, const mda_hop* hop,
+                     const double* Z, int32_t z_on_device, mda_enkf_diag* diag);
+
+/* ---- ETKF (ETKF.hpp:86-178) ---------------------------------------------------- */
+int mda_etkf_analyse(mda_ctx* ctx, double inflation, mda_ensemble* ens, const mda_obs* obs,
+                     const mda_hop* hop);
+
+/* ---- LETKF (LETKF.hpp:47-243) --------------------------------------------------- */
+enum { MDA_LOC_GASPARI_COHN = 0, MDA_LOC_CUTOFF = 1 };
+enum { MDA_RW_REAL = 0, MDA_RW_IDENTITY = 1 };
+enum { MDA_INFLFORM_PRIOR = 0, MDA_INFLFORM_POSTERIOR = 1 };
+enum { MDA_SQRT_SYMMETRIC = 0, MDA_SQRT_CHOLESKY = 1 };
+enum { MDA_COMBINE_MATRIX = 0, MDA_COMBINE_ELEMENTWISE = 1 };
+enum { MDA_PREC_FP64 = 0, MDA_PREC_FP32 = 1, MDA_PREC_BF16X3 = 2 };
+
+typedef struct {
+  double inflation;          /* rho (LETKF.hpp:52, float-narrowed by the config layer) */
+  double radius;             /* localisation radius, units of the distance (LETKF.hpp:53) */
+  int32_t localization;      /* GC taper with support = radius | hard cutoff (LETKF.hpp:162) */
+  int32_t rweight;           /* real R^-1 | R = I (LETKF.hpp:214) */
+  int32_t inflation_form;    /* A=(k-1)I/rho+C | Pa = rho (C+(k-1)I)^-1 (LETKF.hpp:219) */
+  int32_t sqrt_form;         /* symmetric A^-1/2 | chol_L(Pa) (LETKF.hpp:223) */
+  int32_t combine;           /* X'(wa 1^T + Wa) | x'_i (wa_i + sum_j Wa_ij) (LETKF.hpp:235-238) */
+  int32_t precision;         /* contractions in FP64 | FP32 | BF16x3 (eig always FP64) */
+  const double* rho_column;  /* optional per-column rho [nx*ny] (learned inflation) */
+  int32_t rho_on_device;
+} mda_letkf_params;
+
+/* Presets: Hunt et al. (2007) with Gaspari-Cohn R-localisation, and the
+ * reference's own LETKF semantics (snapshot H). */
+void mda_letkf_params_standard(mda_letkf_params* prm, double inflation, double radius);
+void mda_letkf_params_refcompat(mda_letkf_params* prm, double inflation, double radius);
+
+typedef struct {
+  int64_t n_columns;
+  int64_t n_empty_columns;
+  int64_t total_local_obs;
+  int64_t max_local_obs;
+  int32_t max_jacobi_sweeps;
+  double ms_obs;        /* HX, obs-space mean/anomalies, innovations */
+  double ms_bucket;     /* local-observation index */
+  double ms_gram;       /* per-column Y'^T R^-1 Y', Y'^T R^-1 d */
+  double ms_solve;      /* per-column eigensolve / Cholesky and T */
+  double ms_transform;  /* X_a = xbar + X'T */
+  double ms_total;
+} mda_letkf_diag;
+
+int mda_letkf_analyse(mda_ctx* ctx, const mda_letkf_params* prm, mda_ensemble* ens,
+                      const mda_geometry* geom, const mda_obs* obs, const mda_hop* hop,
+                      mda_letkf_diag* diag);
+
+/* Per-column k x k transforms T (X_a = xbar + X' T) of the last LETKF call,
+ * for the listed columns, row-major [ncols][k][k] (host).  Diagnostics and
+ * parity tests; valid until the next call on the context. */
+int mda_letkf_last_transforms(mda_ctx* ctx, const int64_t* cols, int64_t ncols, double* T_out);
+
+/* ---- Metrics (Metrics.hpp:74-294) ------------------------------------------------ */
+typedef struct {
+  double rmse, bias, correlation, crps, avg_spread;
+} mda_metrics;
+int mda_ensemble_metrics(mda_ctx* ctx, const mda_ensemble* ens, const double* truth,
+                         int32_t truth_on_device, mda_metrics* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,14 @@
+"""metada_amd: MI355X (gfx950) ensemble Kalman analysis (EnKF / ETKF / LETKF).
+
+The arithmetic lives in libmetada_amd.so (HIP kernels behind the C-ABI in
+include/metada_amd.h); this package is the host-side mirror of metada's
+analysis interface on top of it.  Importing it does not touch the GPU.
+"""
+from . import _lib
+from .analysis import (ColumnGeometry, Config, ConfigValue, Context, EnKF, Ensemble, ETKF,
+                       LETKF, ObsOperator, Observation, default_context, idw4_stencil,
+                       metrics, read_simple_state)
+
+__all__ = ["ColumnGeometry", "Config", "ConfigValue", "Context", "EnKF", "Ensemble", "ETKF",
+           "LETKF", "ObsOperator", "Observation", "default_context", "idw4_stencil", "metrics",
+           "read_simple_state", "_lib"]

@@ -1,0 +1,143 @@
+"""ctypes binding of libmetada_amd.so (include/metada_amd.h).
+
+The structures below mirror the C declarations field by field.  Loading
+fails loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmetada_amd.so")
+
+MDA_OK = 0
+ERRORS = {-1: "MDA_E_ARG", -2: "MDA_E_DEVICE", -3: "MDA_E_NOMEM", -4: "MDA_E_NUMERIC",
+          -5: "MDA_E_UNSUPPORTED"}
+
+COORD_GRID, COORD_GEOGRAPHIC, COORD_CARTESIAN = 0, 1, 2
+HOP_CSR, HOP_HX = 0, 1
+INFL_MULTIPLICATIVE, INFL_ADDITIVE, INFL_RELAXATION = 0, 1, 2
+LOC_GASPARI_COHN, LOC_CUTOFF = 0, 1
+RW_REAL, RW_IDENTITY = 0, 1
+INFLFORM_PRIOR, INFLFORM_POSTERIOR = 0, 1
+SQRT_SYMMETRIC, SQRT_CHOLESKY = 0, 1
+COMBINE_MATRIX, COMBINE_ELEMENTWISE = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+
+_d, _i32, _i64 = C.c_double, C.c_int32, C.c_int64
+_pd = C.POINTER(C.c_double)
+_pi64 = C.POINTER(C.c_int64)
+
+
+class Ensemble(C.Structure):
+    _fields_ = [("k", _i32), ("nx", _i64), ("ny", _i64), ("nlev", _i64), ("data", C.c_void_p),
+                ("members", C.c_void_p), ("on_device", _i32)]
+
+
+class Obs(C.Structure):
+    _fields_ = [("p", _i64), ("value", C.c_void_p), ("variance", C.c_void_p),
+                ("loc_a", C.c_void_p), ("loc_b", C.c_void_p), ("loc_c", C.c_void_p),
+                ("on_device", _i32)]
+
+
+class Hop(C.Structure):
+    _fields_ = [("kind", _i32), ("row_ptr", C.c_void_p), ("col", C.c_void_p),
+                ("weight", C.c_void_p), ("nnz", _i64), ("hx", C.c_void_p), ("on_device", _i32)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("coord", _i32), ("col_a", C.c_void_p), ("col_b", C.c_void_p),
+                ("col_c", C.c_void_p), ("on_device", _i32)]
+
+
+class EnkfDiag(C.Structure):
+    _fields_ = [("innovation_norm", _d), ("analysis_increment_norm", _d),
+                ("background_spread", _d), ("analysis_spread", _d), ("max_kalman_gain", _d),
+                ("min_kalman_gain", _d), ("condition_number", _d), ("ensemble_size", _i32),
+                ("observation_count", _i64), ("inflation_method", _i32),
+                ("inflation_factor", _d)]
+
+
+class LetkfParams(C.Structure):
+    _fields_ = [("inflation", _d), ("radius", _d), ("localization", _i32), ("rweight", _i32),
+                ("inflation_form", _i32), ("sqrt_form", _i32), ("combine", _i32),
+                ("precision", _i32), ("rho_column", C.c_void_p), ("rho_on_device", _i32)]
+
+
+class LetkfDiag(C.Structure):
+    _fields_ = [("n_columns", _i64), ("n_empty_columns", _i64), ("total_local_obs", _i64),
+                ("max_local_obs", _i64), ("max_jacobi_sweeps", _i32), ("ms_obs", _d),
+                ("ms_bucket", _d), ("ms_gram", _d), ("ms_solve", _d), ("ms_transform", _d),
+                ("ms_total", _d)]
+
+
+class Metrics(C.Structure):
+    _fields_ = [("rmse", _d), ("bias", _d), ("correlation", _d), ("crps", _d),
+                ("avg_spread", _d)]
+
+
+# every exported symbol with its signature; tests check that the library
+# exports exactly what include/metada_amd.h declares
+SIGNATURES = {
+    "mda_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mda_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "mda_last_error": (C.c_char_p, [C.c_void_p]),
+    "mda_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "mda_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "mda_ctx_workspace_bytes": (_i64, [C.c_void_p]),
+    "mda_ensemble_mean_perturbations": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), _d,
+                                                  C.c_void_p, C.c_void_p, _i32]),
+    "mda_obs_innovations": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), C.POINTER(Obs),
+                                      C.POINTER(Hop), C.c_void_p, C.c_void_p, C.c_void_p,
+                                      _i32]),
+    "mda_enkf_analyse": (C.c_int, [C.c_void_p, _d, _i32, C.POINTER(Ensemble), C.POINTER(Obs),
+                                   C.POINTER(Hop), C.c_void_p, _i32, C.POINTER(EnkfDiag)]),
+    "mda_etkf_analyse": (C.c_int, [C.c_void_p, _d, C.POINTER(Ensemble), C.POINTER(Obs),
+                                   C.POINTER(Hop)]),
+    "mda_letkf_params_standard": (None, [C.POINTER(LetkfParams), _d, _d]),
+    "mda_letkf_params_refcompat": (None, [C.POINTER(LetkfParams), _d, _d]),
+    "mda_letkf_analyse": (C.c_int, [C.c_void_p, C.POINTER(LetkfParams), C.POINTER(Ensemble),
+                                    C.POINTER(Geometry), C.POINTER(Obs), C.POINTER(Hop),
+                                    C.POINTER(LetkfDiag)]),
+    "mda_letkf_last_transforms": (C.c_int, [C.c_void_p, C.c_void_p, _i64, C.c_void_p]),
+    "mda_ensemble_metrics": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), C.c_void_p, _i32,
+                                       C.POINTER(Metrics)]),
+}
+
+_lib = None
+
+
+class MdaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load():
+    """Loads the built library; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        # torch bundles its own HIP runtime with the same SONAME
+        # (libamdhip64.so.7); loading torch first makes this library bind to
+        # that runtime, so one process never holds two HIP runtimes.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m metada_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(ctx, rc):
+    if rc != MDA_OK:
+        msg = load().mda_last_error(ctx)
+        raise MdaError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,657 @@
+// LETKF kernels (LETKF.hpp:152-243 re-designed per column):
+//   1. observation bucketing: key = bucket of the observation location, stable
+//      LSD radix sort, bucket start offsets, gather of the observation data
+//      into bucket order;
+//   2. per-column Gram C_g = sum_o w_go/var_o y'_o y'_o^T and b_g on FP64 MFMA
+//      (one wavefront per column, candidates from the buckets within reach);
+//   3. per-column k x k solve: parallel Jacobi (symmetric square root) or
+//      Cholesky (reference square root) -> transform T_g;
+//   4. X_a = xbar + X' T_g for every level of the column.
+#include "mda_device.hpp"
+#include "mda_internal.hpp"
+
+namespace mda {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// 1. bucketing
+// ---------------------------------------------------------------------------
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ inline double norm_lon(double lon) {
+  double l = fmod(lon, 360.0);
+  return l < 0.0 ? l + 360.0 : l;
+}
+
+__device__ inline uint32_t bucket_of(const bucket_grid& g, double r, double c) {
+  if (g.coord == MDA_COORD_GEOGRAPHIC) c = norm_lon(c);
+  int row = clampi((int)floor((r - g.a0) / g.da), 0, g.nrow - 1);
+  int col = clampi((int)floor((c - g.b0) / g.db), 0, g.ncolb - 1);
+  return (uint32_t)row * (uint32_t)g.ncolb + (uint32_t)col;
+}
+
+// rcoord: latitude (GEOGRAPHIC) or y; ccoord: longitude or x
+__global__ void k_bucket_keys(const double* __restrict__ rcoord,
+                              const double* __restrict__ ccoord, int64_t p, bucket_grid g,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < p;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    keys[o] = bucket_of(g, rcoord[o], ccoord[o]);
+    vals[o] = (uint32_t)o;
+  }
+}
+
+void launch_bucket_keys(hipStream_t s, const double* rcoord, const double* ccoord, int64_t p,
+                        bucket_grid g, uint32_t* keys, uint32_t* vals) {
+  hipLaunchKernelGGL(k_bucket_keys, dim3(grid_for(p, 256, 8192)), dim3(256), 0, s, rcoord,
+                     ccoord, p, g, keys, vals);
+  MDA_LAUNCH_CHECK();
+}
+
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits.  Per pass: block
+// digit histograms (digit-major), one exclusive scan, stable scatter with
+// in-wave ranks from ballots.  Deterministic.
+constexpr int RS_BLOCK = 256;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict__ keys,
+                                                      int64_t p, int shift,
+                                                      uint32_t* __restrict__ hist,
+                                                      int64_t nblocks) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t i = blockIdx.x * (int64_t)RS_BLOCK + threadIdx.x;
+  if (i < p) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of `total` uint32 in place, one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ h, int64_t total) {
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  int64_t chunk = (total + 1023) / 1024;
+  int64_t lo = tid * chunk, hi = lo + chunk < total ? lo + chunk : total;
+  uint32_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += h[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[tid] - s;  // exclusive prefix of this chunk
+  for (int64_t i = lo; i < hi; ++i) {
+    uint32_t v = h[i];
+    h[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t p, int shift,
+    const uint32_t* __restrict__ hist, int64_t nblocks) {
+  __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int t = tid; t < (RS_BLOCK / 64) * 256; t += RS_BLOCK) (&wcnt[0][0])[t] = 0;
+  __syncthreads();
+  int64_t i = blockIdx.x * (int64_t)RS_BLOCK + tid;
+  bool valid = i < p;
+  uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+  uint32_t dig = (key >> shift) & 255u;
+  uint64_t match = __ballot(valid);
+  for (int b = 0; b < 8; ++b) {
+    uint64_t m = __ballot(valid && ((dig >> b) & 1u));
+    match &= ((dig >> b) & 1u) ? m : ~m;
+  }
+  uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t rank = (uint32_t)__popcll(match & below);
+  if (valid && rank == 0) wcnt[wv][dig] = (uint32_t)__popcll(match);
+  __syncthreads();
+  if (valid) {
+    uint32_t off = 0;
+    for (int w = 0; w < wv; ++w) off += wcnt[w][dig];
+    uint32_t dst = hist[(size_t)dig * nblocks + blockIdx.x] + off + rank;
+    kout[dst] = key;
+    vout[dst] = val;
+  }
+}
+
+void radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
+                      uint32_t* vals_tmp, int64_t p, int key_bits, uint32_t* hist,
+                      int64_t hist_cap) {
+  if (p <= 0) return;
+  int64_t nblocks = (p + RS_BLOCK - 1) / RS_BLOCK;
+  MDA_REQUIRE(nblocks * 256 <= hist_cap, "radix histogram workspace too small");
+  uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+  int passes = 0;
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nblocks), dim3(RS_BLOCK), 0, s, ki, p, shift,
+                       hist, nblocks);
+    MDA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, hist, nblocks * 256);
+    MDA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nblocks), dim3(RS_BLOCK), 0, s, ki, vi, ko,
+                       vo, p, shift, hist, nblocks);
+    MDA_LAUNCH_CHECK();
+    uint32_t* t = ki; ki = ko; ko = t;
+    t = vi; vi = vo; vo = t;
+    ++passes;
+  }
+  if (passes & 1) {  // result sits in the tmp buffers: copy back
+    MDA_HIP(hipMemcpyAsync(keys, ki, p * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    MDA_HIP(hipMemcpyAsync(vals, vi, p * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  }
+}
+
+// start[b] = first sorted position whose key >= b, for b in [0, nbuckets]
+__global__ void k_bucket_ranges(const uint32_t* __restrict__ sk, int64_t p, int nbuckets,
+                                int* __restrict__ start) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= p;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int lo = (t == 0) ? 0 : (int)sk[t - 1] + 1;
+    int hi = (t == p) ? nbuckets : (int)sk[t];
+    for (int b = lo; b <= hi; ++b) start[b] = (int)t;
+  }
+}
+
+void launch_bucket_ranges(hipStream_t s, const uint32_t* sorted_keys, int64_t p, int nbuckets,
+                          int* start) {
+  hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(p + 1, 256, 8192)), dim3(256), 0, s,
+                     sorted_keys, p, nbuckets, start);
+  MDA_LAUNCH_CHECK();
+}
+
+__global__ void k_gather_obs(const uint32_t* __restrict__ order, int64_t p, int k,
+                             const double* __restrict__ yp_om, const double* __restrict__ d,
+                             const double* __restrict__ var, const double* __restrict__ oa,
+                             const double* __restrict__ ob, const double* __restrict__ oc,
+                             int rweight, double* __restrict__ s_yp, double* __restrict__ s_d,
+                             double* __restrict__ s_rinv, double* __restrict__ s_a,
+                             double* __restrict__ s_b, double* __restrict__ s_c) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < p * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = e / k;
+    int j = (int)(e % k);
+    uint32_t o = order[t];
+    s_yp[e] = yp_om[(size_t)o * k + j];
+    if (j == 0) {
+      s_d[t] = d[o];
+      s_rinv[t] = rweight == MDA_RW_IDENTITY ? 1.0 : 1.0 / var[o];
+      s_a[t] = oa[o];
+      s_b[t] = ob[o];
+      s_c[t] = oc ? oc[o] : 0.0;
+    }
+  }
+}
+
+void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, int k,
+                       const double* yp_om, const double* d, const double* var,
+                       const double* oa, const double* ob, const double* oc, int rweight,
+                       double* s_yp, double* s_d, double* s_rinv, double* s_a, double* s_b,
+                       double* s_c) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_gather_obs, dim3(grid_for(p * k, 256, 65536)), dim3(256), 0, s, order,
+                     p, k, yp_om, d, var, oa, ob, oc, rweight, s_yp, s_d, s_rinv, s_a, s_b,
+                     s_c);
+  MDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 2. per-column Gram on FP64 MFMA.  One wavefront per column.  For each
+// bucket range in reach, 64 candidates at a time: distance (Location.hpp
+// metric), localisation weight (GC taper or cutoff) times 1/var, ballot-
+// compaction of the nonzero weights into LDS, then K-steps of 4 observations
+// through v_mfma_f64_16x16x4f64 on the lower 16x16 tiles of C.
+//   f64 16x16x4 lane layout: A[l%16][l/16], B[l/16][l%16],
+//   D[l/16 + 4r][l%16] (r = 0..3).
+// ---------------------------------------------------------------------------
+struct seg_list {
+  int n;
+  int j0[2], j1[2];
+};
+
+__device__ inline seg_list col_segments(const bucket_grid& g, double c_lo, double c_hi,
+                                        bool full) {
+  seg_list s;
+  if (full || (g.wrap && c_hi - c_lo >= 360.0 - g.db)) {
+    s.n = 1; s.j0[0] = 0; s.j1[0] = g.ncolb - 1;
+    return s;
+  }
+  const int last = g.ncolb - 1;
+  if (!g.wrap) {
+    s.n = 1;
+    s.j0[0] = clampi((int)floor((c_lo - g.b0) / g.db), 0, last);
+    s.j1[0] = clampi((int)floor((c_hi - g.b0) / g.db), 0, last);
+    return s;
+  }
+  // longitude buckets tile [0, 360) exactly (db = 360 / ncolb): the part of
+  // the range beyond either end is bucketed from its wrapped longitude
+  if (c_lo < 0.0) {
+    s.n = 2;
+    s.j0[0] = clampi((int)floor((c_lo + 360.0) / g.db), 0, last); s.j1[0] = last;
+    s.j0[1] = 0; s.j1[1] = clampi((int)floor(c_hi / g.db), 0, last);
+  } else if (c_hi >= 360.0) {
+    s.n = 2;
+    s.j0[0] = clampi((int)floor(c_lo / g.db), 0, last); s.j1[0] = last;
+    s.j0[1] = 0; s.j1[1] = clampi((int)floor((c_hi - 360.0) / g.db), 0, last);
+  } else {
+    s.n = 1;
+    s.j0[0] = clampi((int)floor(c_lo / g.db), 0, last);
+    s.j1[0] = clampi((int)floor(c_hi / g.db), 0, last);
+  }
+  return s;
+}
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int KT>
+__global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
+  constexpr int NT = KT * (KT + 1) / 2;
+  __shared__ int q_off[4][64];
+  __shared__ double q_w[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const int k = a.k;
+  const bucket_grid& G = a.grid;
+  const double cgc = a.radius / 2.0;
+  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
+    d4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    double bp[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) bp[t] = 0.0;
+    int nl = 0;
+    // column location and rows of buckets in reach
+    double ca, cb, cc = 0.0;
+    if (a.col_a) { ca = a.col_a[g]; cb = a.col_b[g]; }
+    else { ca = (double)(g % a.nx); cb = (double)(g / a.nx); }
+    if (a.col_c) cc = a.col_c[g];
+    double r_lo, r_hi, c_lo, c_hi;
+    bool full = false;
+    double cos_lat = 0.0;
+    if (a.coord == MDA_COORD_GEOGRAPHIC) {
+      const double dd = a.radius / kEarthRadiusKm;
+      const double ddeg = dd * 180.0 / kPi + 1e-9;
+      r_lo = ca - ddeg; r_hi = ca + ddeg;
+      cos_lat = cos(ca * kPi / 180.0);
+      full = r_lo <= -90.0 || r_hi >= 90.0 || sin(dd) >= cos_lat;
+      double dl = full ? 180.0 : asin(fmin(1.0, sin(dd) / cos_lat)) * 180.0 / kPi + 1e-9;
+      double lon = norm_lon(cb);
+      c_lo = lon - dl; c_hi = lon + dl;
+    } else {  // GRID / CARTESIAN: rows along y, bucket columns along x
+      r_lo = cb - a.radius - 1e-9; r_hi = cb + a.radius + 1e-9;
+      c_lo = ca - a.radius - 1e-9; c_hi = ca + a.radius + 1e-9;
+    }
+    int row0 = clampi((int)floor((r_lo - G.a0) / G.da), 0, G.nrow - 1);
+    int row1 = clampi((int)floor((r_hi - G.a0) / G.da), 0, G.nrow - 1);
+    seg_list sl = col_segments(G, c_lo, c_hi, full);
+    for (int row = row0; row <= row1; ++row) {
+      for (int sg = 0; sg < sl.n; ++sg) {
+        int64_t s0 = a.bstart[row * G.ncolb + sl.j0[sg]];
+        int64_t s1 = a.bstart[row * G.ncolb + sl.j1[sg] + 1];
+        for (int64_t base = s0; base < s1; base += 64) {
+          int64_t t = base + lane;
+          double w = 0.0;
+          bool in = false;
+          if (t < s1) {
+            double dist;
+            if (a.coord == MDA_COORD_GEOGRAPHIC) {
+              dist = haversine_km(ca, cb, a.oa[t], a.ob[t]);
+            } else if (a.coord == MDA_COORD_CARTESIAN) {
+              double dx = ca - a.oa[t], dy = cb - a.ob[t], dz = cc - a.oc[t];
+              dist = sqrt(dx * dx + dy * dy + dz * dz);
+            } else {
+              double dx = ca - a.oa[t], dy = cb - a.ob[t];
+              dist = sqrt(dx * dx + dy * dy);
+            }
+            in = dist <= a.radius;
+            if (in)
+              w = (a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / cgc) : 1.0) *
+                  a.rinv[t];
+          }
+          nl += __popcll(__ballot(in));
+          uint64_t wm = __ballot(w != 0.0);
+          int cnt = __popcll(wm);
+          if (cnt == 0) continue;
+          uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+          int rank = __popcll(wm & below);
+          if (w != 0.0) { q_off[wv][rank] = lane; q_w[wv][rank] = w; }
+          wave_sync_lds();
+          for (int s = 0; s < cnt; s += 4) {
+            int tq = s + kk;
+            double wt = 0.0;
+            int64_t oi = -1;
+            if (tq < cnt) { wt = q_w[wv][tq]; oi = base + q_off[wv][tq]; }
+            double yv[KT];
+#pragma unroll
+            for (int ta = 0; ta < KT; ++ta) {
+              int e = 16 * ta + i16;
+              yv[ta] = (oi >= 0 && e < k) ? a.yp[(size_t)oi * k + e] : 0.0;
+            }
+            double dv = oi >= 0 ? a.d[oi] : 0.0;
+            int tile = 0;
+#pragma unroll
+            for (int ta = 0; ta < KT; ++ta) {
+              double aw = wt * yv[ta];
+              bp[ta] += aw * dv;
+#pragma unroll
+              for (int tb = 0; tb <= ta; ++tb) {
+                acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, yv[tb], acc[tile], 0, 0, 0);
+                ++tile;
+              }
+            }
+          }
+          wave_sync_lds();
+        }
+      }
+    }
+    // b: sum the four observation slots (lanes i16, i16+16, i16+32, i16+48)
+#pragma unroll
+    for (int ta = 0; ta < KT; ++ta) {
+      bp[ta] += __shfl_xor(bp[ta], 16);
+      bp[ta] += __shfl_xor(bp[ta], 32);
+    }
+    double* Cg = a.C + (size_t)g * k * k;
+    int tile = 0;
+#pragma unroll
+    for (int ta = 0; ta < KT; ++ta) {
+#pragma unroll
+      for (int tb = 0; tb <= ta; ++tb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = 16 * ta + kk + 4 * r, col = 16 * tb + i16;
+          if (row < k && col < k) {
+            Cg[(size_t)row * k + col] = acc[tile][r];
+            if (ta != tb) Cg[(size_t)col * k + row] = acc[tile][r];
+          }
+        }
+        ++tile;
+      }
+    }
+    if (kk == 0) {
+#pragma unroll
+      for (int ta = 0; ta < KT; ++ta) {
+        int e = 16 * ta + i16;
+        if (e < k) a.b[(size_t)g * k + e] = bp[ta];
+      }
+    }
+    if (lane == 0) a.nloc[g] = nl;
+  }
+}
+
+void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
+  unsigned nb = (unsigned)((a.ncol + 3) / 4);
+  if (nb > 262144) nb = 262144;
+  int kt = (a.k + 15) / 16;
+  switch (kt) {
+    case 1: hipLaunchKernelGGL(k_letkf_gram_f64<1>, dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_letkf_gram_f64<2>, dim3(nb), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_letkf_gram_f64<3>, dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_letkf_gram_f64<4>, dim3(nb), dim3(256), 0, s, a); break;
+    default: throw mda_error(MDA_E_UNSUPPORTED, "LETKF Gram supports k <= 64 in this build");
+  }
+  MDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 3. per-column solve.  With sigma = (k-1)/rho (prior inflation, Hunt et al.
+// 2007) or k-1 (posterior: Pa *= rho, LETKF.hpp:219) and s = 1 or rho:
+//   M = C + sigma I,  Pa = s M^-1,  wbar = Pa b,
+//   W = sqrt(k-1) Pa^(1/2)  (symmetric: eigen-decomposition of M by Jacobi;
+//                            Cholesky: chol_L(Pa), LETKF.hpp:223-224),
+//   T = wbar 1^T + W  (matrix combination) or diag(wbar + W 1) (the
+//   element-wise combination of LETKF.hpp:235-238).
+// A column without local observations gets T = sqrt(rho) I (LETKF.hpp:167-190).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double* gscratch,
+                                                     int use_lds) {
+  extern __shared__ double sh[];
+  __shared__ double rc[64], rs[64], red[256], bv[128], yv[128], wb[128];
+  __shared__ int rp[64], rq[64], flag;
+  const int k = a.k, tid = threadIdx.x, nt = blockDim.x;
+  const int kp = (k + 1) & ~1, ld = kp + 1;
+  double* A = use_lds ? sh : gscratch + (size_t)blockIdx.x * 2 * kp * ld;
+  double* V = A + kp * ld;
+  for (int64_t g = blockIdx.x; g < a.ncol; g += gridDim.x) {
+    const double rho = a.rho_col ? a.rho_col[g] : a.rho;
+    double* Tg = a.T + (size_t)g * k * k;
+    if (a.nloc[g] == 0) {
+      const double sr = sqrt(rho);
+      for (int idx = tid; idx < k * k; idx += nt) Tg[idx] = (idx / k == idx % k) ? sr : 0.0;
+      if (tid == 0 && a.sweeps) a.sweeps[g] = 0;
+      continue;
+    }
+    const bool prior = a.inflation_form == MDA_INFLFORM_PRIOR;
+    const double sigma = prior ? (double)(k - 1) / rho : (double)(k - 1);
+    const double sc = prior ? 1.0 : rho;
+    const double* Cg = a.C + (size_t)g * k * k;
+    for (int idx = tid; idx < kp * kp; idx += nt) {
+      int i = idx / kp, j = idx % kp;
+      double v;
+      if (i < k && j < k) v = Cg[i * k + j] + (i == j ? sigma : 0.0);
+      else v = (i == j) ? 1.0 : 0.0;  // decoupled pad index (odd k)
+      A[i * ld + j] = v;
+    }
+    for (int i = tid; i < k; i += nt) bv[i] = a.b[(size_t)g * k + i];
+    __syncthreads();
+    const double sq = sqrt((double)(k - 1));
+    if (a.sqrt_form == MDA_SQRT_SYMMETRIC) {
+      int sw = block_jacobi(A, ld, V, ld, kp, rc, rs, rp, rq, red, 40, 1e-15);
+      if (tid == 0 && a.sweeps) a.sweeps[g] = sw;
+      // y = L^-1 V^T b ; wbar = s V y
+      for (int e = tid; e < k; e += nt) {
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += V[i * ld + e] * bv[i];
+        yv[e] = s / A[e * ld + e];
+      }
+      __syncthreads();
+      for (int i = tid; i < k; i += nt) {
+        double s = 0.0;
+        for (int e = 0; e < k; ++e) s += V[i * ld + e] * yv[e];
+        wb[i] = sc * s;
+      }
+      __syncthreads();
+      // U = V L^-1/4 so that W = sqrt(k-1) sqrt(s) U U^T
+      for (int idx = tid; idx < k * k; idx += nt) {
+        int i = idx / k, e = idx % k;
+        V[i * ld + e] *= 1.0 / sqrt(sqrt(A[e * ld + e]));
+      }
+      __syncthreads();
+      const double coef = sq * sqrt(sc);
+      if (a.combine == MDA_COMBINE_MATRIX) {
+        for (int idx = tid; idx < k * k; idx += nt) {
+          int i = idx / k, j = idx % k;
+          double s = 0.0;
+          for (int e = 0; e < k; ++e) s += V[i * ld + e] * V[j * ld + e];
+          Tg[idx] = wb[i] + coef * s;
+        }
+      } else {
+        for (int i = tid; i < k; i += nt) {
+          double rsum = 0.0;
+          for (int j = 0; j < k; ++j) {
+            double s = 0.0;
+            for (int e = 0; e < k; ++e) s += V[i * ld + e] * V[j * ld + e];
+            rsum += coef * s;
+          }
+          yv[i] = wb[i] + rsum;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k * k; idx += nt)
+          Tg[idx] = (idx / k == idx % k) ? yv[idx / k] : 0.0;
+      }
+    } else {
+      bool ok = block_cholesky(A, ld, k, &flag);
+      block_spd_inverse_from_chol(A, ld, V, ld, k);  // A <- M^-1
+      for (int idx = tid; idx < k * k; idx += nt) A[(idx / k) * ld + idx % k] *= sc;
+      __syncthreads();
+      for (int i = tid; i < k; i += nt) {
+        double s = 0.0;
+        for (int c = 0; c < k; ++c) s += A[i * ld + c] * bv[c];
+        wb[i] = s;
+      }
+      __syncthreads();
+      ok = block_cholesky(A, ld, k, &flag) && ok;  // A <- chol_L(Pa)
+      if (tid == 0 && a.status && !ok) atomicOr(a.status, 1);
+      if (a.combine == MDA_COMBINE_MATRIX) {
+        for (int idx = tid; idx < k * k; idx += nt) {
+          int i = idx / k, j = idx % k;
+          Tg[idx] = wb[i] + sq * A[i * ld + j];
+        }
+      } else {
+        for (int i = tid; i < k; i += nt) {
+          double rsum = 0.0;
+          for (int j = 0; j < k; ++j) rsum += sq * A[i * ld + j];
+          yv[i] = wb[i] + rsum;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < k * k; idx += nt)
+          Tg[idx] = (idx / k == idx % k) ? yv[idx / k] : 0.0;
+      }
+      if (tid == 0 && a.sweeps) a.sweeps[g] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a) {
+  MDA_REQUIRE(a.k >= 2 && a.k <= 128, "LETKF solve supports 2 <= k <= 128");
+  const int kp = (a.k + 1) & ~1, ld = kp + 1;
+  size_t shm = sizeof(double) * 2 * kp * ld;
+  int use_lds = shm <= 96 * 1024;
+  unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
+  if (nb < 1) nb = 1;
+  double* gscratch = nullptr;
+  if (!use_lds) throw mda_error(MDA_E_UNSUPPORTED, "LETKF solve for k > 96 not in this build");
+  hipLaunchKernelGGL(k_letkf_solve, dim3(nb), dim3(256), use_lds ? shm : 0, s, a, gscratch,
+                     use_lds);
+  MDA_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 4. X_a = xbar + X' T_g for a tile of 16 consecutive columns x 8 levels: the
+// tile is staged through LDS with coalesced 128-byte rows, the mean is formed
+// per (level, column) as in Ensemble::RecomputeMean, and the product runs out
+// of LDS with T_g streamed from global memory.
+// ---------------------------------------------------------------------------
+constexpr int TR_C = 16, TR_L = 8;
+
+__global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X, int k,
+                                                         int64_t ncol, int64_t nlev,
+                                                         const double* __restrict__ T) {
+  extern __shared__ double xs[];  // [TR_L][TR_C][k+1]
+  __shared__ double xm[TR_L][TR_C];
+  const int tid = threadIdx.x, nt = blockDim.x, ld = k + 1;
+  const int64_t n = ncol * nlev;
+  const int64_t g0 = (int64_t)blockIdx.x * TR_C, l0 = (int64_t)blockIdx.y * TR_L;
+  const int ncl = (int)(ncol - g0 < TR_C ? ncol - g0 : TR_C);
+  const int nll = (int)(nlev - l0 < TR_L ? nlev - l0 : TR_L);
+  for (int idx = tid; idx < k * TR_L * TR_C; idx += nt) {
+    int c = idx % TR_C, l = (idx / TR_C) % TR_L, m = idx / (TR_C * TR_L);
+    if (c < ncl && l < nll) xs[(l * TR_C + c) * ld + m] = X[(size_t)m * n + (l0 + l) * ncol + g0 + c];
+  }
+  __syncthreads();
+  const double rk = 1.0 / (double)k;
+  for (int idx = tid; idx < TR_L * TR_C; idx += nt) {
+    int l = idx / TR_C, c = idx % TR_C;
+    if (c < ncl && l < nll) {
+      double* row = xs + (l * TR_C + c) * ld;
+      double s = 0.0;
+      for (int m = 0; m < k; ++m) s += row[m];
+      double mean = s * rk;
+      xm[l][c] = mean;
+      for (int m = 0; m < k; ++m) row[m] -= mean;
+    }
+  }
+  __syncthreads();
+  constexpr int QMAX = 8;
+  double acc[QMAX][TR_L];
+  const int nq = (TR_C * k + nt - 1) / nt;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q)
+#pragma unroll
+    for (int l = 0; l < TR_L; ++l) acc[q][l] = 0.0;
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (q >= nq) break;
+    int e = tid + q * nt;
+    int c = e / k, j = e % k;
+    if (c < ncl) {
+      const double* Tg = T + (size_t)(g0 + c) * k * k;
+      for (int m = 0; m < k; ++m) {
+        double t = Tg[(size_t)m * k + j];
+#pragma unroll
+        for (int l = 0; l < TR_L; ++l) acc[q][l] += xs[(l * TR_C + c) * ld + m] * t;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    if (q >= nq) break;
+    int e = tid + q * nt;
+    int c = e / k, j = e % k;
+    if (c < ncl)
+#pragma unroll
+      for (int l = 0; l < TR_L; ++l)
+        if (l < nll) xs[(l * TR_C + c) * ld + j] = xm[l][c] + acc[q][l];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < k * TR_L * TR_C; idx += nt) {
+    int c = idx % TR_C, l = (idx / TR_C) % TR_L, m = idx / (TR_C * TR_L);
+    if (c < ncl && l < nll) X[(size_t)m * n + (l0 + l) * ncol + g0 + c] = xs[(l * TR_C + c) * ld + m];
+  }
+}
+
+void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
+                            const double* T) {
+  MDA_REQUIRE(k <= 128, "transform supports k <= 128");
+  size_t shm = sizeof(double) * TR_L * TR_C * (k + 1);
+  dim3 grid((unsigned)((ncol + TR_C - 1) / TR_C), (unsigned)((nlev + TR_L - 1) / TR_L));
+  hipLaunchKernelGGL(k_letkf_transform, grid, dim3(256), shm, s, X, k, ncol, nlev, T);
+  MDA_LAUNCH_CHECK();
+}
+
+// nloc statistics: out[0] = sum, out[1] = max, out[2] = #zeros
+__global__ void k_reduce_int(const int* __restrict__ v, int64_t n, int64_t* out) {
+  __shared__ long long s_sum[256], s_max[256], s_zero[256];
+  long long su = 0, mx = 0, ze = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    long long x = v[i];
+    su += x; mx = x > mx ? x : mx; ze += (x == 0);
+  }
+  s_sum[threadIdx.x] = su; s_max[threadIdx.x] = mx; s_zero[threadIdx.x] = ze;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
+      s_max[threadIdx.x] = s_max[threadIdx.x] > s_max[threadIdx.x + s] ? s_max[threadIdx.x]
+                                                                       : s_max[threadIdx.x + s];
+      s_zero[threadIdx.x] += s_zero[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&out[0], (unsigned long long)s_sum[0]);
+    atomicMax((long long*)&out[1], s_max[0]);
+    atomicAdd((unsigned long long*)&out[2], (unsigned long long)s_zero[0]);
+  }
+}
+
+void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out3,
+                       int64_t* /*unused*/) {
+  hipLaunchKernelGGL(k_reduce_int, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, v, n, out3);
+  MDA_LAUNCH_CHECK();
+}
+
+}  // namespace mda

@@ -1,0 +1,208 @@
+// Internal declarations shared by the HIP sources of libmetada_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "metada_amd.h"
+
+// ---------------------------------------------------------------------------
+// errors: every entry point converts these into MDA_E_* codes
+// ---------------------------------------------------------------------------
+struct mda_error : std::runtime_error {
+  int code;
+  mda_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MDA_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw mda_error(e_ == hipErrorOutOfMemory ? MDA_E_NOMEM : MDA_E_DEVICE,      \
+                      std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define MDA_REQUIRE(cond, msg)                                                     \
+  do {                                                                             \
+    if (!(cond)) throw mda_error(MDA_E_ARG, msg);                                  \
+  } while (0)
+
+#define MDA_LAUNCH_CHECK() MDA_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct mda_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  struct buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, buf> ws;  // named device workspace, grown on demand
+  hipEvent_t ev[8] = {};
+  // state of the last LETKF call (for mda_letkf_last_transforms)
+  int last_k = 0;
+  int64_t last_ncol = 0;
+  double* last_T = nullptr;  // device, [ncol][k][k]
+
+  // returns a device buffer of at least `bytes`; contents are unspecified
+  void* get(const std::string& name, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    buf& b = ws[name];
+    if (b.bytes < bytes) {
+      if (b.ptr) MDA_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+      size_t want = bytes + bytes / 8;  // headroom: fewer re-allocations
+      MDA_HIP(hipMalloc(&b.ptr, want));
+      b.bytes = want;
+    }
+    return b.ptr;
+  }
+  template <class T>
+  T* get(const std::string& name, size_t count) {
+    return static_cast<T*>(get(name, count * sizeof(T)));
+  }
+};
+
+// Scoped "make this context's device current".
+struct device_guard {
+  int prev = 0;
+  explicit device_guard(int dev) {
+    MDA_HIP(hipGetDevice(&prev));
+    if (prev != dev) MDA_HIP(hipSetDevice(dev));
+  }
+  ~device_guard() { (void)hipSetDevice(prev); }
+};
+
+// Host data staged to the device (no-op when already on the device).
+template <class T>
+const T* to_device(mda_ctx* c, const std::string& name, const T* src, size_t count,
+                   bool on_device) {
+  if (src == nullptr) return nullptr;
+  if (on_device) return src;
+  T* d = c->get<T>(name, count);
+  MDA_HIP(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return d;
+}
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// kernels launched by the API layer (defined in the kernel sources)
+// ---------------------------------------------------------------------------
+namespace mda {
+
+// clears `bytes` (multiple of 4) with a kernel, so the clear is graph-safe
+void launch_zero(hipStream_t s, void* ptr, size_t bytes);
+// streaming ensemble statistics
+void launch_mean_pert(hipStream_t s, const double* X, int k, int64_t n, double scale,
+                      double* mean, double* pert);
+void launch_sumsq(hipStream_t s, const double* X, int64_t count, double* partial,
+                  int nblocks);
+void launch_dev_sumsq(hipStream_t s, const double* X, int k, int64_t n, double scale,
+                      double* partial, int nblocks);
+// observation space: Y' obs-major [p][k], ybar, d; optional member-major Y'
+void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t p,
+                      const int64_t* row_ptr, const int64_t* col, const double* w,
+                      const double* hx_mm, const double* yo, double* ybar, double* yp_om,
+                      double* d, double* yp_mm);
+// global (all-observation) Gram: G[a][c] = sum_o r_o Y'[o][a] Q[o][c] with Q =
+// [Y' | d] (mode 0) or D = d + sigma Z - Y' (mode 1, EnKF); partials reduced.
+void launch_global_gram(hipStream_t s, const double* yp_om, const double* d,
+                        const double* var, const double* Z, int k, int64_t p, int mode,
+                        double* partial, int nblocks, double* out);
+// small dense solves on one workgroup (matrices in global scratch)
+void launch_etkf_solve_ws(hipStream_t s, int k, const double* EB, double* T, double* scratch,
+                          int* status);
+void launch_enkf_solve_ws(hipStream_t s, int k, const double* EB, const double* F, double* T,
+                          double* Minv, double* scratch, int* status);
+// X_a = xbar + scale * X' T for all state points (global T)
+void launch_transform_global(hipStream_t s, double* X, int k, int64_t n, double scale,
+                             const double* T);
+// EnKF diagnostics
+void launch_enkf_gain_minmax_ws(hipStream_t s, const double* X, int k, int64_t n, double scale,
+                                const double* Minv, const double* yp_om, const double* var,
+                                int64_t p, double* G, double* partial, int nblocks);
+void launch_cond_small(hipStream_t s, const double* yp_om, const double* var, int k, int p,
+                       double* S, double* out);
+void launch_sym_maxeig(hipStream_t s, const double* EB, int k, double* S, double* out);
+void launch_metrics(hipStream_t s, const double* X, int k, int64_t n, const double* truth,
+                    double* partial, int nblocks);
+
+// LETKF
+struct bucket_grid {
+  int coord;
+  int nrow, ncolb;       // bucket rows (lat / y) and bucket columns (lon / x)
+  double a0, b0;         // origin of the bucket grid
+  double da, db;         // bucket sizes
+  int wrap;              // longitude wrap-around
+};
+void launch_bucket_keys(hipStream_t s, const double* oa, const double* ob, int64_t p,
+                        bucket_grid g, uint32_t* keys, uint32_t* vals);
+void radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
+                      uint32_t* vals_tmp, int64_t p, int key_bits, uint32_t* hist,
+                      int64_t hist_cap);
+void launch_bucket_ranges(hipStream_t s, const uint32_t* sorted_keys, int64_t p,
+                          int nbuckets, int* start);
+void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, int k,
+                       const double* yp_om, const double* d, const double* var,
+                       const double* oa, const double* ob, const double* oc, int rweight,
+                       double* s_yp, double* s_d, double* s_rinv, double* s_a, double* s_b,
+                       double* s_c);
+struct letkf_gram_args {
+  int k;
+  int64_t ncol, nx;
+  int coord;
+  const double* col_a;
+  const double* col_b;
+  const double* col_c;
+  bucket_grid grid;
+  const int* bstart;
+  const double* yp;    // sorted obs-major Y'
+  const double* d;     // sorted innovations
+  const double* rinv;  // sorted 1/var (or 1)
+  const double* oa;
+  const double* ob;
+  const double* oc;
+  double radius;
+  int localization;
+  int precision;
+  double* C;    // out [ncol][k][k]
+  double* b;    // out [ncol][k]
+  int* nloc;    // out [ncol]
+};
+void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
+struct letkf_solve_args {
+  int k;
+  int64_t ncol;
+  const double* C;
+  const double* b;
+  const int* nloc;
+  double rho;
+  const double* rho_col;
+  int inflation_form, sqrt_form, combine;
+  double* T;
+  int* sweeps;
+  int* status;
+};
+void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
+void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
+                            const double* T);
+void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,
+                       int64_t* out_empty);
+
+}  // namespace mda

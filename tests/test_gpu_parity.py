@@ -1,0 +1,235 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the
+same seeded inputs.  FP64 gate: 1e-10 relative (max-norm)."""
+import os
+
+import numpy as np
+import pytest
+
+from cases import l96_case, latlon_case, rel_err
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def mda():
+    import metada_amd
+    return metada_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(mda):
+    c = mda.Context(0)
+    yield c
+    c.close()
+
+
+def test_mean_perturbations_match_reference_order(mda, ctx, orc):
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((13, 5001)) * 3 + 1
+    ens = mda.Ensemble(X, 5001, 1, 1)
+    m = ens.RecomputeMean(ctx)
+    np.testing.assert_array_equal(m, orc.ensemble_mean(X))       # same summation order
+    P = ens.Perturbations(np.sqrt(1.05), ctx)
+    np.testing.assert_allclose(P, orc.ensemble_perturbations(X, m, np.sqrt(1.05)), rtol=1e-15,
+                               atol=1e-15)
+
+
+def test_obs_innovations_csr_and_hx(mda, ctx, orc):
+    import ctypes as C
+    case = latlon_case(nlon=32, nlat=16, nlev=3, k=11, p=777)
+    X, H = case["X"], case["H"]
+    ens = mda.Ensemble(X, 32, 16, 3)
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"])
+    from metada_amd.analysis import _obs_struct
+    es, ek = ens.struct(); os_, ok = _obs_struct(obs); hs, hk = H.struct()
+    k, p = 11, 777
+    ybar, yp, d = np.zeros(p), np.zeros((k, p)), np.zeros(p)
+    ctx.check(ctx.lib.mda_obs_innovations(ctx.h, C.byref(es), C.byref(os_), C.byref(hs),
+                                          ybar.ctypes.data, yp.ctypes.data, d.ctypes.data, 0))
+    Yb = np.stack([orc.csr_apply(H.row_ptr, H.col, H.weight, X[j]) for j in range(k)])
+    assert rel_err(ybar, Yb.mean(0)) < 1e-14
+    assert rel_err(yp, Yb - Yb.mean(0)) < 1e-13
+    assert rel_err(d, case["yo"] - Yb.mean(0)) < 1e-13
+    # caller-supplied H(x)
+    H2 = mda.ObsOperator(hx=Yb)
+    hs2, hk2 = H2.struct()
+    yp2 = np.zeros((k, p))
+    ctx.check(ctx.lib.mda_obs_innovations(ctx.h, C.byref(es), C.byref(os_), C.byref(hs2),
+                                          None, yp2.ctypes.data, None, 0))
+    assert rel_err(yp2, Yb - Yb.mean(0)) < 1e-14
+
+
+def _enkf_setup(mda, case):
+    ens = mda.Ensemble(case["X"].copy(), case["N"], 1, 1)
+    obs = mda.Observation(case["yo"], case["var"], case["obs_x"], case["obs_y"])
+    H = mda.ObsOperator(case["rp"], case["cols"], case["ws"])
+    return ens, obs, H
+
+
+def test_enkf_l96_matches_oracle(mda, ctx, orc):
+    case = l96_case(orc)
+    ens, obs, H = _enkf_setup(mda, case)
+    cfg = mda.Config({"analysis": {"inflation": 1.05, "inflation_method": "multiplicative",
+                                   "output_base_file": "x", "format": "txt"}})
+    f = mda.EnKF(ens, obs, H, cfg, ctx=ctx)
+    f.Analyse(Z=case["Z"])
+    Yb = np.stack([orc.csr_apply(case["rp"], case["cols"], case["ws"], x) for x in case["X"]])
+    Xo, dg = orc.enkf_analyse(case["X"], Yb, case["yo"], case["var"], case["Z"],
+                              f.inflation_factor)
+    assert f.inflation_factor == 1.0499999523162842  # float-narrowed as the reference
+    assert rel_err(ens.data, Xo) < TOL
+    res = f.getAnalysisResults()
+    for key in ("innovation_norm", "background_spread", "analysis_spread",
+                "max_kalman_gain", "min_kalman_gain", "condition_number"):
+        assert abs(res[key] - dg[key]) <= TOL * abs(dg[key]), key
+    assert res["analysis_increment_norm"] == 0.0
+    assert res["ensemble_size"] == 20 and res["observation_count"] == 20
+
+
+def test_enkf_large_p_constant_R_condition(mda, ctx, orc):
+    # p > 128 with R = sigma^2 I: cond(S) from the k x k eigenproblem
+    rng = np.random.default_rng(3)
+    k, n, p = 10, 400, 300
+    X = rng.standard_normal((k, n))
+    cols = rng.integers(0, n, p)
+    rp = np.arange(p + 1); ws = np.ones(p)
+    yo = rng.standard_normal(p); var = np.full(p, 0.7); Z = rng.standard_normal((p, k))
+    ens = mda.Ensemble(X.copy(), n, 1, 1)
+    obs = mda.Observation(yo, var, np.zeros(p), np.zeros(p))
+    f = mda.EnKF(ens, obs, mda.ObsOperator(rp, cols, ws),
+                 mda.Config({"analysis": {"inflation": 1.0}}), ctx=ctx)
+    f.Analyse(Z=Z)
+    Yb = X[:, cols]
+    Xo, dg = orc.enkf_analyse(X, Yb, yo, var, Z, 1.0)
+    assert rel_err(ens.data, Xo) < TOL
+    assert abs(f.getAnalysisResults()["condition_number"] - dg["condition_number"]) < 1e-9 * dg["condition_number"]
+
+
+def test_etkf_matches_oracle(mda, ctx, orc):
+    case = l96_case(orc, k=16, seed=9)
+    ens, obs, H = _enkf_setup(mda, case)
+    f = mda.ETKF(ens, obs, H, mda.Config({"inflation": 1.1}), ctx=ctx)
+    f.Analyse()
+    Yb = np.stack([orc.csr_apply(case["rp"], case["cols"], case["ws"], x) for x in case["X"]])
+    Xo = orc.etkf_analyse(case["X"], Yb, case["yo"], case["var"], f.inflation)
+    assert rel_err(ens.data, Xo) < TOL
+
+
+def _letkf_run(mda, ctx, case, radius, rho, form="standard", device=False, cols_T=None):
+    nlon, nlat, nlev = case["nlon"], case["nlat"], case["nlev"]
+    X = case["X"].copy()
+    if device:
+        import torch
+        X = torch.from_numpy(X).cuda()
+    ens = mda.Ensemble(X, nlon, nlat, nlev)
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                          coord=mda._lib.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+    cfg = mda.Config({"inflation": rho, "localization_radius": radius})
+    f = mda.LETKF(ens, obs, case["H"], cfg, geometry=geom, ctx=ctx, form=form)
+    diag = f.Analyse()
+    T = f.transforms(cols_T) if cols_T is not None else None
+    return ens.to_host(), f, diag, T
+
+
+def _oracle_problem(orc, case, f):
+    return orc.Problem(case["k"], case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+
+
+@pytest.mark.parametrize("k", [20, 64, 7])
+def test_letkf_standard_matches_oracle(mda, ctx, orc, k):
+    case = latlon_case(k=k, p=3000 if k != 7 else 800, seed=11 + k)
+    cols = np.array([0, 5, 100, 1000, 2047])
+    Xg, f, diag, Tg = _letkf_run(mda, ctx, case, 2000.0, 1.05, cols_T=cols)
+    prob = _oracle_problem(orc, case, f)
+    Yb = case["H"].apply_host(case["X"])
+    Xo, To, nloc = orc.letkf_standard(prob, case["X"], Yb, want_T=True)
+    assert rel_err(Xg, Xo) < TOL
+    assert rel_err(Tg, To[cols]) < TOL
+    assert diag["total_local_obs"] == nloc.sum()
+    assert diag["max_local_obs"] == nloc.max()
+    assert diag["n_empty_columns"] == int((nloc == 0).sum())
+
+
+def test_letkf_standard_device_tensor_in_place(mda, ctx, orc):
+    case = latlon_case(k=20, p=2500, seed=5, cluster=True)
+    Xg, f, diag, _ = _letkf_run(mda, ctx, case, 1500.0, 1.1, device=True)
+    prob = _oracle_problem(orc, case, f)
+    Xo, _, nloc = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert diag["n_empty_columns"] > 0  # clustered obs leave columns without local obs
+    assert rel_err(Xg, Xo) < TOL
+
+
+def test_letkf_refcompat_tutorial_matches_oracle(mda, ctx, orc):
+    """The reference's tutorial LETKF case (36 x 18 grid, 9 members, 28 grid
+    obs, radius 10, inflation 1.0; src/backends/simple/tutorial/letkf.yaml)
+    in the reference semantics with snapshot H."""
+    tut = os.path.join(ROOT, "tests", "golden", "tutorial")
+    X = np.stack([mda.read_simple_state(os.path.join(tut, f"ens_{i}.txt")) for i in range(1, 10)])
+    obs = mda.Observation.from_grid_file(os.path.join(tut, "obs_format.txt"), error=np.float32(0.1))
+    H = mda.ObsOperator.identity_idw4(obs, 36, 18, 1)
+    cfg = mda.Config({"inflation": 1.0, "localization_radius": 10.0})
+    ens = mda.Ensemble(X.copy(), 36, 18, 1)
+    f = mda.LETKF(ens, obs, H, cfg, ctx=ctx, form="refcompat")
+    f.Analyse()
+    prob = orc.Problem(9, 36, 18, 1, orc.COORD_GRID,
+                       np.tile(np.arange(36.0), 18), np.repeat(np.arange(18.0), 36),
+                       obs.loc_a, obs.loc_b, obs.values, obs.variances, 1.0, 10.0)
+    Xo = orc.letkf_refcompat_snapshot(prob, X, H.apply_host(X))
+    assert rel_err(ens.data, Xo) < TOL
+
+
+def test_letkf_refcompat_latlon_with_inflation(mda, ctx, orc):
+    case = latlon_case(k=16, p=1500, seed=21)
+    Xg, f, diag, _ = _letkf_run(mda, ctx, case, 1200.0, 1.2, form="refcompat")
+    prob = _oracle_problem(orc, case, f)
+    Xo = orc.letkf_refcompat_snapshot(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert rel_err(Xg, Xo) < TOL
+
+
+def test_letkf_grid_coordinates_and_per_column_rho(mda, ctx, orc):
+    rng = np.random.default_rng(4)
+    nx, ny, nlev, k, p = 24, 17, 2, 12, 150
+    X = rng.standard_normal((k, nx * ny * nlev))
+    ox, oy = rng.uniform(0, nx - 1, p), rng.uniform(0, ny - 1, p)
+    oz = rng.integers(0, nlev, p).astype(float)
+    obs = mda.Observation(rng.standard_normal(p), rng.uniform(0.2, 2.0, p), ox, oy, oz)
+    H = mda.ObsOperator.identity_idw4(obs, nx, ny, nlev)
+    rho_col = rng.uniform(1.0, 1.3, nx * ny)
+    ens = mda.Ensemble(X.copy(), nx, ny, nlev)
+    f = mda.LETKF(ens, obs, H, mda.Config({"inflation": 1.05, "localization_radius": 4.5}),
+                  ctx=ctx, rho_column=rho_col)
+    f.Analyse()
+    prob = orc.Problem(k, nx, ny, nlev, orc.COORD_GRID, np.tile(np.arange(nx, dtype=float), ny),
+                       np.repeat(np.arange(ny, dtype=float), nx), ox, oy, obs.values,
+                       obs.variances, f.inflation, f.localization_radius, rho_col=rho_col)
+    Xo, _, _ = orc.letkf_standard(prob, X, H.apply_host(X))
+    assert rel_err(ens.data, Xo) < TOL
+
+
+def test_metrics_match_oracle(mda, ctx, orc):
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((9, 3000)); t = rng.standard_normal(3000)
+    g = mda.metrics(mda.Ensemble(X, 3000, 1, 1), t, ctx)
+    o = orc.metrics(X, t)
+    for key in ("rmse", "bias", "correlation", "crps", "avg_spread"):
+        assert abs(g[key] - o[key]) <= 1e-12 * max(1.0, abs(o[key])), key
+
+
+def test_error_paths(mda, ctx):
+    from metada_amd._lib import MdaError
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((4, 10))
+    obs = mda.Observation(np.zeros(2), np.ones(2), np.zeros(2), np.zeros(2))
+    bad_H = mda.ObsOperator(np.array([0, 1, 2]), np.array([0, 99]), np.ones(2))
+    with pytest.raises(MdaError, match="MDA_E_ARG"):
+        mda.ETKF(mda.Ensemble(X.copy(), 10, 1, 1), obs, bad_H, mda.Config({"inflation": 1.0}),
+                 ctx=ctx).Analyse()
+    H = mda.ObsOperator(np.array([0, 1, 2]), np.array([0, 3]), np.ones(2))
+    with pytest.raises(MdaError, match="MDA_E_ARG"):
+        mda.LETKF(mda.Ensemble(X.copy(), 10, 1, 1), obs, H,
+                  mda.Config({"inflation": 1.0, "localization_radius": -1.0}), ctx=ctx).Analyse()
