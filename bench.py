@@ -90,6 +90,8 @@ def main():
     cpu = None
     T_oracle = None
     try:
+        if a.no_cpu_baseline:
+            raise RuntimeError("skipped (--no-cpu-baseline)")
         from oracle import oracle as orc
         threads = a.threads or len(os.sched_getaffinity(0))
         prob = orc.Problem(k, len(cols), 1, nlev, orc.COORD_GEOGRAPHIC, col_lat[cols],
@@ -129,9 +131,13 @@ def main():
             parity = {"oracle": "standard LETKF (C, FP64)", "columns": len(cols),
                       "max_rel_err_T": err}
             log(f"parity on {len(cols)} sampled columns: max rel err of T = {err:.3e}")
+        if d0["n_fallback_columns"]:
+            fl = f.column_flags()
+            log("fallback reasons: " + ", ".join(f"{nm}={int(((fl & bit) != 0).sum())}" for bit, nm in
+                ((1, "ql"), (2, "residual"), (4, "orthogonality"), (8, "below_sigma"))))
         log(f"warmup {w}: {d0['ms_total']:.1f} ms  (gram {d0['ms_gram']:.1f}, solve "
             f"{d0['ms_solve']:.1f}, transform {d0['ms_transform']:.1f}, obs {d0['ms_obs']:.1f},"
-            f" bucket {d0['ms_bucket']:.1f})")
+            f" bucket {d0['ms_bucket']:.1f}; fallback columns {d0['n_fallback_columns']})")
     torch.cuda.synchronize()
     stages = {s: 0.0 for s in ("ms_obs", "ms_bucket", "ms_gram", "ms_solve", "ms_transform")}
     t_start = time.perf_counter()
@@ -178,6 +184,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_ms": stages,
+        "eig_fallback_columns": d["n_fallback_columns"],
         "local_obs": {"total": tot_loc, "mean": tot_loc / ncol, "max": d["max_local_obs"],
                       "empty_columns": d["n_empty_columns"]},
         "parity": parity,

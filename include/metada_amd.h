@@ -165,7 +165,14 @@ typedef struct {
   int32_t precision;         /* contractions in FP64 | FP32 | BF16x3 (eig always FP64) */
   const double* rho_column;  /* optional per-column rho [nx*ny] (learned inflation) */
   int32_t rho_on_device;
+  int32_t eig_method;        /* symmetric square root: MDA_EIG_AUTO (Householder
+                                tridiagonalisation, QL eigenvalues, twisted-
+                                factorisation eigenvectors; columns failing the
+                                residual/orthogonality checks are re-solved by
+                                Jacobi) | MDA_EIG_JACOBI (parallel-order Jacobi) */
 } mda_letkf_params;
+
+enum { MDA_EIG_AUTO = 0, MDA_EIG_JACOBI = 1 };
 
 /* Presets: Hunt et al. (2007) with Gaspari-Cohn R-localisation, and the
  * reference's own LETKF semantics (snapshot H). */
@@ -178,6 +185,7 @@ typedef struct {
   int64_t total_local_obs;
   int64_t max_local_obs;
   int32_t max_jacobi_sweeps;
+  int64_t n_fallback_columns; /* columns the tridiagonal path handed to Jacobi */
   double ms_obs;        /* HX, obs-space mean/anomalies, innovations */
   double ms_bucket;     /* local-observation index */
   double ms_gram;       /* per-column Y'^T R^-1 Y', Y'^T R^-1 d */
@@ -194,6 +202,17 @@ int mda_letkf_analyse(mda_ctx* ctx, const mda_letkf_params* prm, mda_ensemble* e
  * for the listed columns, row-major [ncols][k][k] (host).  Diagnostics and
  * parity tests; valid until the next call on the context. */
 int mda_letkf_last_transforms(mda_ctx* ctx, const int64_t* cols, int64_t ncols, double* T_out);
+
+/* Number of local observations (within the localisation radius) of every
+ * column in the last LETKF call, [nx*ny] (host).  The p_g histogram of a
+ * workload; the reference builds the same sets at LETKF.hpp:158-165. */
+int mda_letkf_last_local_counts(mda_ctx* ctx, int32_t* counts);
+
+/* Per-column eigensolver status of the last LETKF call, [nx*ny] (host):
+ * 0 = tridiagonal path, otherwise the column was re-solved by Jacobi and the
+ * bits say why (1 QL non-convergence, 2 eigen-residual, 4 orthogonality of a
+ * close eigenpair, 8 eigenvalue below (k-1)/rho). */
+int mda_letkf_last_column_flags(mda_ctx* ctx, int32_t* flags);
 
 /* ---- Metrics (Metrics.hpp:74-294) ------------------------------------------------ */
 typedef struct {

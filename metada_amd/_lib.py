@@ -24,6 +24,7 @@ INFLFORM_PRIOR, INFLFORM_POSTERIOR = 0, 1
 SQRT_SYMMETRIC, SQRT_CHOLESKY = 0, 1
 COMBINE_MATRIX, COMBINE_ELEMENTWISE = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+EIG_AUTO, EIG_JACOBI = 0, 1
 
 _d, _i32, _i64 = C.c_double, C.c_int32, C.c_int64
 _pd = C.POINTER(C.c_double)
@@ -62,12 +63,14 @@ class EnkfDiag(C.Structure):
 class LetkfParams(C.Structure):
     _fields_ = [("inflation", _d), ("radius", _d), ("localization", _i32), ("rweight", _i32),
                 ("inflation_form", _i32), ("sqrt_form", _i32), ("combine", _i32),
-                ("precision", _i32), ("rho_column", C.c_void_p), ("rho_on_device", _i32)]
+                ("precision", _i32), ("rho_column", C.c_void_p), ("rho_on_device", _i32),
+                ("eig_method", _i32)]
 
 
 class LetkfDiag(C.Structure):
     _fields_ = [("n_columns", _i64), ("n_empty_columns", _i64), ("total_local_obs", _i64),
-                ("max_local_obs", _i64), ("max_jacobi_sweeps", _i32), ("ms_obs", _d),
+                ("max_local_obs", _i64), ("max_jacobi_sweeps", _i32),
+                ("n_fallback_columns", _i64), ("ms_obs", _d),
                 ("ms_bucket", _d), ("ms_gram", _d), ("ms_solve", _d), ("ms_transform", _d),
                 ("ms_total", _d)]
 
@@ -101,6 +104,8 @@ SIGNATURES = {
                                     C.POINTER(Geometry), C.POINTER(Obs), C.POINTER(Hop),
                                     C.POINTER(LetkfDiag)]),
     "mda_letkf_last_transforms": (C.c_int, [C.c_void_p, C.c_void_p, _i64, C.c_void_p]),
+    "mda_letkf_last_local_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mda_letkf_last_column_flags": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mda_ensemble_metrics": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), C.c_void_p, _i32,
                                        C.POINTER(Metrics)]),
 }

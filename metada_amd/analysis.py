@@ -528,7 +528,7 @@ class LETKF(_AlgorithmBase):
 
     def __init__(self, ensemble, obs, obs_op, config: Config, geometry: ColumnGeometry = None,
                  ctx=None, form: str = "standard", precision: int = L.PREC_FP64,
-                 rho_column=None):
+                 rho_column=None, eig_method: int = L.EIG_AUTO):
         super().__init__(ensemble, obs, obs_op, ctx)
         self.inflation = config.Get("inflation").asFloat()
         self.localization_radius = config.Get("localization_radius").asFloat()
@@ -545,6 +545,7 @@ class LETKF(_AlgorithmBase):
         else:
             raise ValueError(f"unknown LETKF form {form!r}")
         self.params.precision = precision
+        self.params.eig_method = eig_method
         self._rho_keep = None
         if rho_column is not None:
             rp, rdev, self._rho_keep = _buf(rho_column)
@@ -571,6 +572,22 @@ class LETKF(_AlgorithmBase):
         self.ctx.check(self.ctx.lib.mda_letkf_last_transforms(self.ctx.h, cols.ctypes.data,
                                                               len(cols), out.ctypes.data))
         return out
+
+
+def _local_counts(self) -> np.ndarray:
+    out = np.empty(self.ensemble.nx * self.ensemble.ny, np.int32)
+    self.ctx.check(self.ctx.lib.mda_letkf_last_local_counts(self.ctx.h, out.ctypes.data))
+    return out
+
+
+def _column_flags(self) -> np.ndarray:
+    out = np.empty(self.ensemble.nx * self.ensemble.ny, np.int32)
+    self.ctx.check(self.ctx.lib.mda_letkf_last_column_flags(self.ctx.h, out.ctypes.data))
+    return out
+
+
+LETKF.local_counts = _local_counts
+LETKF.column_flags = _column_flags
 
 
 def metrics(ensemble: Ensemble, truth, ctx: Optional[Context] = None) -> dict:

@@ -50,14 +50,12 @@ dev_ensemble stage_ensemble(mda_ctx* c, const mda_ensemble* e) {
   d.X = c->get<double>("ens", (size_t)d.k * d.n);
   d.staged = true;
   if (e->data) {
-    MDA_HIP(hipMemcpyAsync(d.X, e->data, sizeof(double) * d.k * d.n, hipMemcpyHostToDevice,
-                           c->stream));
+    h2d(c, d.X, e->data, sizeof(double) * d.k * d.n);
   } else {
     MDA_REQUIRE(e->members != nullptr, "ensemble needs data or members");
     for (int j = 0; j < d.k; ++j) {
       MDA_REQUIRE(e->members[j] != nullptr, "member pointer is null");
-      MDA_HIP(hipMemcpyAsync(d.X + (size_t)j * d.n, e->members[j], sizeof(double) * d.n,
-                             hipMemcpyHostToDevice, c->stream));
+      h2d(c, d.X + (size_t)j * d.n, e->members[j], sizeof(double) * d.n);
     }
   }
   return d;
@@ -66,12 +64,10 @@ dev_ensemble stage_ensemble(mda_ctx* c, const mda_ensemble* e) {
 void unstage_ensemble(mda_ctx* c, const mda_ensemble* e, const dev_ensemble& d) {
   if (!d.staged) return;
   if (e->data) {
-    MDA_HIP(hipMemcpyAsync(e->data, d.X, sizeof(double) * d.k * d.n, hipMemcpyDeviceToHost,
-                           c->stream));
+    d2h(c, e->data, d.X, sizeof(double) * d.k * d.n);
   } else {
     for (int j = 0; j < d.k; ++j)
-      MDA_HIP(hipMemcpyAsync(e->members[j], d.X + (size_t)j * d.n, sizeof(double) * d.n,
-                             hipMemcpyDeviceToHost, c->stream));
+      d2h(c, e->members[j], d.X + (size_t)j * d.n, sizeof(double) * d.n);
   }
   MDA_HIP(hipStreamSynchronize(c->stream));
 }
@@ -137,7 +133,7 @@ dev_hop stage_hop(mda_ctx* c, const mda_hop* h, int k, int64_t p, int64_t n) {
 
 double host_sum(mda_ctx* c, const double* dev, int n) {
   std::vector<double> h(n);
-  MDA_HIP(hipMemcpyAsync(h.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  d2h(c, h.data(), dev, sizeof(double) * n);
   MDA_HIP(hipStreamSynchronize(c->stream));
   double s = 0.0;
   for (double v : h) s += v;
@@ -219,9 +215,9 @@ int mda_ensemble_mean_perturbations(mda_ctx* c, const mda_ensemble* ens, double 
     double* dp = pert ? (out_on_device ? pert : c->get<double>("out_pert", d.k * d.n)) : nullptr;
     launch_mean_pert(c->stream, d.X, d.k, d.n, scale, dm, dp);
     if (!out_on_device) {
-      if (mean) MDA_HIP(hipMemcpyAsync(mean, dm, sizeof(double) * d.n, hipMemcpyDeviceToHost, c->stream));
+      if (mean) d2h(c, mean, dm, sizeof(double) * d.n);
       if (pert)
-        MDA_HIP(hipMemcpyAsync(pert, dp, sizeof(double) * d.k * d.n, hipMemcpyDeviceToHost, c->stream));
+        d2h(c, pert, dp, sizeof(double) * d.k * d.n);
     }
     MDA_HIP(hipStreamSynchronize(c->stream));
   });
@@ -241,9 +237,9 @@ int mda_obs_innovations(mda_ctx* c, const mda_ensemble* ens, const mda_obs* obs,
     launch_obs_stats(c->stream, de.X, de.k, de.n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, dyb,
                      nullptr, dd, dyp);
     if (!out_on_device) {
-      if (ybar) MDA_HIP(hipMemcpyAsync(ybar, dyb, sizeof(double) * p, hipMemcpyDeviceToHost, c->stream));
-      if (yp) MDA_HIP(hipMemcpyAsync(yp, dyp, sizeof(double) * de.k * p, hipMemcpyDeviceToHost, c->stream));
-      if (d) MDA_HIP(hipMemcpyAsync(d, dd, sizeof(double) * p, hipMemcpyDeviceToHost, c->stream));
+      if (ybar) d2h(c, ybar, dyb, sizeof(double) * p);
+      if (yp) d2h(c, yp, dyp, sizeof(double) * de.k * p);
+      if (d) d2h(c, d, dd, sizeof(double) * p);
     }
     MDA_HIP(hipStreamSynchronize(c->stream));
   });
@@ -273,7 +269,7 @@ int mda_etkf_analyse(mda_ctx* c, double inflation, mda_ensemble* ens, const mda_
     launch_etkf_solve_ws(c->stream, k, EB, T, scr, st);
     launch_transform_global(c->stream, de.X, k, de.n, inflation, T);
     int hs = 0;
-    MDA_HIP(hipMemcpyAsync(&hs, st, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    d2h(c, &hs, st, sizeof(int));
     MDA_HIP(hipStreamSynchronize(c->stream));
     if (hs) throw mda_error(MDA_E_NUMERIC, "ETKF: Pa is not positive definite");
     unstage_ensemble(c, ens, de);
@@ -328,7 +324,7 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
         double* mm = c->get<double>("enkf_mm", 2 * 512);
         launch_enkf_gain_minmax_ws(c->stream, de.X, k, n, scale, Minv, yp, o.var, p, G, mm, 512);
         std::vector<double> hm(2 * 512);
-        MDA_HIP(hipMemcpyAsync(hm.data(), mm, sizeof(double) * 1024, hipMemcpyDeviceToHost, c->stream));
+        d2h(c, hm.data(), mm, sizeof(double) * 1024);
         MDA_HIP(hipStreamSynchronize(c->stream));
         double mx = -std::numeric_limits<double>::infinity(), mn = -mx;
         for (int b = 0; b < 512; ++b) { mx = std::max(mx, hm[2 * b]); mn = std::min(mn, hm[2 * b + 1]); }
@@ -342,11 +338,11 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
       if (p <= 128) {
         double* S = c->get<double>("enkf_S", (size_t)130 * 130);
         launch_cond_small(c->stream, yp, o.var, k, (int)p, S, cnd);
-        MDA_HIP(hipMemcpyAsync(&dg.condition_number, cnd, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        d2h(c, &dg.condition_number, cnd, sizeof(double));
         MDA_HIP(hipStreamSynchronize(c->stream));
       } else {
         std::vector<double> hv(p);
-        MDA_HIP(hipMemcpyAsync(hv.data(), o.var, sizeof(double) * p, hipMemcpyDeviceToHost, c->stream));
+        d2h(c, hv.data(), o.var, sizeof(double) * p);
         MDA_HIP(hipStreamSynchronize(c->stream));
         bool constant = std::all_of(hv.begin(), hv.end(), [&](double v) { return v == hv[0]; });
         if (constant) {
@@ -354,7 +350,7 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
           double* S = c->get<double>("enkf_S", (size_t)130 * 130);
           launch_sym_maxeig(c->stream, EB, k, S, cnd);
           double mu = 0.0;
-          MDA_HIP(hipMemcpyAsync(&mu, cnd, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+          d2h(c, &mu, cnd, sizeof(double));
           MDA_HIP(hipStreamSynchronize(c->stream));
           dg.condition_number = 1.0 + mu / (double)(k - 1);
         } else {
@@ -364,7 +360,7 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
     }
     launch_transform_global(c->stream, de.X, k, n, scale, T);
     int hs = 0;
-    MDA_HIP(hipMemcpyAsync(&hs, st, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    d2h(c, &hs, st, sizeof(int));
     if (diag) {
       launch_dev_sumsq(c->stream, de.X, k, n, 1.0, part2, 512);
       dg.analysis_spread = std::sqrt(host_sum(c, part2, 512) / ((double)n * k));
@@ -423,9 +419,9 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     const int64_t ncol = nx * ny;
     MDA_REQUIRE(k <= 64, "LETKF supports k <= 64 in this build");
     const bool gd = geom->on_device != 0;
-    const double* ca = to_device(c, "col_a", geom->col_a, ncol, gd);
-    const double* cb = to_device(c, "col_b", geom->col_b, ncol, gd);
-    const double* cc = to_device(c, "col_c", geom->col_c, ncol, gd);
+    const double* ca = to_device(c, "geom_a", geom->col_a, ncol, gd);
+    const double* cb = to_device(c, "geom_b", geom->col_b, ncol, gd);
+    const double* cc = to_device(c, "geom_c", geom->col_c, ncol, gd);
     MDA_REQUIRE(geom->coord == MDA_COORD_GRID || (ca && cb),
                 "GEOGRAPHIC / CARTESIAN geometry needs column coordinates");
     const double* rho_col =
@@ -457,8 +453,8 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
       double xmin = 0, xmax = (double)(nx - 1), ymin = 0, ymax = (double)(ny - 1);
       if (geom->col_a) {
         std::vector<double> hx(ncol), hy(ncol);
-        MDA_HIP(hipMemcpyAsync(hx.data(), ca, sizeof(double) * ncol, hipMemcpyDeviceToHost, s));
-        MDA_HIP(hipMemcpyAsync(hy.data(), cb, sizeof(double) * ncol, hipMemcpyDeviceToHost, s));
+        d2h(c, hx.data(), ca, sizeof(double) * ncol);
+        d2h(c, hy.data(), cb, sizeof(double) * ncol);
         MDA_HIP(hipStreamSynchronize(s));
         auto mx = std::minmax_element(hx.begin(), hx.end());
         auto my = std::minmax_element(hy.begin(), hy.end());
@@ -498,7 +494,7 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
 
     // 3. per-column Gram
     double* C = c->get<double>("col_C", (size_t)ncol * k * k);
-    double* bvec = c->get<double>("col_b", (size_t)ncol * k);
+    double* bvec = c->get<double>("col_bvec", (size_t)ncol * k);
     int* nloc = c->get<int>("col_nloc", ncol);
     letkf_gram_args ga{};
     ga.k = k; ga.ncol = ncol; ga.nx = nx; ga.coord = geom->coord;
@@ -520,6 +516,11 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     sa.rho = prm->inflation; sa.rho_col = rho_col;
     sa.inflation_form = prm->inflation_form; sa.sqrt_form = prm->sqrt_form;
     sa.combine = prm->combine; sa.T = T; sa.sweeps = sweeps; sa.status = status;
+    sa.de = c->get<double>("eig_de", (size_t)ncol * 2 * k);
+    sa.lam = c->get<double>("eig_lam", (size_t)ncol * k);
+    sa.flag = c->get<int>("eig_flag", ncol);
+    sa.method = prm->eig_method;
+    launch_zero(s, sweeps, sizeof(int) * ncol);
     launch_letkf_solve(s, sa);
     MDA_HIP(hipEventRecord(c->ev[5], s));
 
@@ -527,16 +528,20 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     launch_letkf_transform(s, de.X, k, ncol, nlev, T);
     MDA_HIP(hipEventRecord(c->ev[6], s));
 
-    int64_t* stats = c->get<int64_t>("col_stats", 6);
-    launch_zero(s, stats, sizeof(int64_t) * 6);
+    int64_t* stats = c->get<int64_t>("col_stats", 9);
+    launch_zero(s, stats, sizeof(int64_t) * 9);
     launch_reduce_int(s, nloc, ncol, stats, nullptr);
     launch_reduce_int(s, sweeps, ncol, stats + 3, nullptr);
-    int64_t hs[6];
+    const bool eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && k <= 64 &&
+                          prm->eig_method == MDA_EIG_AUTO;
+    if (eig_path) launch_reduce_int(s, sa.flag, ncol, stats + 6, nullptr);
+    int64_t hs[9] = {0};
     int hstat = 0;
-    MDA_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
-    MDA_HIP(hipMemcpyAsync(&hstat, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    d2h(c, hs, stats, sizeof(hs));
+    d2h(c, &hstat, status, sizeof(int));
     MDA_HIP(hipStreamSynchronize(s));
-    c->last_T = T; c->last_k = k; c->last_ncol = ncol;
+    c->last_T = T; c->last_k = k; c->last_ncol = ncol; c->last_nloc = nloc;
+    c->last_flag = eig_path ? sa.flag : nullptr;
     if (hstat) throw mda_error(MDA_E_NUMERIC, "LETKF: a local Pa is not positive definite");
     unstage_ensemble(c, ens, de);
     if (diag) {
@@ -545,6 +550,7 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
       diag->max_local_obs = hs[1];
       diag->n_empty_columns = hs[2];
       diag->max_jacobi_sweeps = (int32_t)hs[4];
+      diag->n_fallback_columns = eig_path ? ncol - hs[8] : 0;  // nonzero flags
       diag->ms_obs = elapsed(c->ev[1], c->ev[2]);
       diag->ms_bucket = elapsed(c->ev[2], c->ev[3]);
       diag->ms_gram = elapsed(c->ev[3], c->ev[4]);
@@ -562,10 +568,28 @@ int mda_letkf_last_transforms(mda_ctx* c, const int64_t* cols, int64_t ncols, do
     const size_t kk = (size_t)c->last_k * c->last_k;
     for (int64_t i = 0; i < ncols; ++i) {
       MDA_REQUIRE(cols[i] >= 0 && cols[i] < c->last_ncol, "column index out of range");
-      MDA_HIP(hipMemcpyAsync(T_out + i * kk, c->last_T + cols[i] * kk, sizeof(double) * kk,
-                             hipMemcpyDeviceToHost, c->stream));
+      d2h(c, T_out + i * kk, c->last_T + cols[i] * kk, sizeof(double) * kk);
     }
     MDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mda_letkf_last_local_counts(mda_ctx* c, int32_t* counts) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(c->last_T != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(counts, "counts is required");
+    d2h(c, counts, c->last_nloc, sizeof(int32_t) * c->last_ncol);
+  });
+}
+
+int mda_letkf_last_column_flags(mda_ctx* c, int32_t* flags) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(c->last_T != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(flags, "flags is required");
+    if (c->last_flag)
+      d2h(c, flags, c->last_flag, sizeof(int32_t) * c->last_ncol);
+    else
+      std::memset(flags, 0, sizeof(int32_t) * c->last_ncol);
   });
 }
 
@@ -579,7 +603,7 @@ int mda_ensemble_metrics(mda_ctx* c, const mda_ensemble* ens, const double* trut
     double* part = c->get<double>("metrics_part", (size_t)nb * 9);
     launch_metrics(c->stream, de.X, de.k, de.n, t, part, nb);
     std::vector<double> h((size_t)nb * 9);
-    MDA_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, c->stream));
+    d2h(c, h.data(), part, sizeof(double) * nb * 9);
     MDA_HIP(hipStreamSynchronize(c->stream));
     double a[9] = {0};
     for (int b = 0; b < nb; ++b)

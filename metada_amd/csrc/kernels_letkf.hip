@@ -414,7 +414,7 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
 // A column without local observations gets T = sqrt(rho) I (LETKF.hpp:167-190).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double* gscratch,
-                                                     int use_lds) {
+                                                     int use_lds, const int* only) {
   extern __shared__ double sh[];
   __shared__ double rc[64], rs[64], red[256], bv[128], yv[128], wb[128];
   __shared__ int rp[64], rq[64], flag;
@@ -423,6 +423,8 @@ __global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double*
   double* A = use_lds ? sh : gscratch + (size_t)blockIdx.x * 2 * kp * ld;
   double* V = A + kp * ld;
   for (int64_t g = blockIdx.x; g < a.ncol; g += gridDim.x) {
+    // fallback mode: flagged columns, plus the empty columns (sqrt(rho) I)
+    if (only && only[g] == 0 && a.nloc[g] != 0) continue;
     const double rho = a.rho_col ? a.rho_col[g] : a.rho;
     double* Tg = a.T + (size_t)g * k * k;
     if (a.nloc[g] == 0) {
@@ -532,8 +534,14 @@ void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a) {
   if (nb < 1) nb = 1;
   double* gscratch = nullptr;
   if (!use_lds) throw mda_error(MDA_E_UNSUPPORTED, "LETKF solve for k > 96 not in this build");
+  const int* only = nullptr;
+  if (a.sqrt_form == MDA_SQRT_SYMMETRIC && a.k <= 64 && a.method == 0) {
+    // tridiagonal eigensolver; flagged columns fall back to Jacobi below
+    launch_letkf_eig(s, a, a.de, a.lam, a.flag);
+    only = a.flag;
+  }
   hipLaunchKernelGGL(k_letkf_solve, dim3(nb), dim3(256), use_lds ? shm : 0, s, a, gscratch,
-                     use_lds);
+                     use_lds, only);
   MDA_LAUNCH_CHECK();
 }
 

@@ -52,8 +52,12 @@ struct mda_ctx {
   int last_k = 0;
   int64_t last_ncol = 0;
   double* last_T = nullptr;  // device, [ncol][k][k]
+  int* last_nloc = nullptr;  // device, [ncol]
+  int* last_flag = nullptr;  // device, [ncol] (tridiagonal path only)
 
-  // returns a device buffer of at least `bytes`; contents are unspecified
+  // returns a device buffer of at least `bytes`; contents are unspecified.
+  // Each name has one meaning; a call fetches a name once, since growing a
+  // buffer frees the previous allocation.
   void* get(const std::string& name, size_t bytes) {
     if (bytes == 0) bytes = 256;
     buf& b = ws[name];
@@ -83,6 +87,19 @@ struct device_guard {
   ~device_guard() { (void)hipSetDevice(prev); }
 };
 
+// Host <-> device transfers.  Host buffers are ordinary pageable memory, so
+// the copy is synchronous: the context's stream is drained first (it is
+// non-blocking, so the copy would not otherwise wait for it) and the copy has
+// completed when the call returns.
+inline void h2d(mda_ctx* c, void* dst, const void* src, size_t bytes) {
+  MDA_HIP(hipStreamSynchronize(c->stream));
+  MDA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+inline void d2h(mda_ctx* c, void* dst, const void* src, size_t bytes) {
+  MDA_HIP(hipStreamSynchronize(c->stream));
+  MDA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
 // Host data staged to the device (no-op when already on the device).
 template <class T>
 const T* to_device(mda_ctx* c, const std::string& name, const T* src, size_t count,
@@ -90,7 +107,7 @@ const T* to_device(mda_ctx* c, const std::string& name, const T* src, size_t cou
   if (src == nullptr) return nullptr;
   if (on_device) return src;
   T* d = c->get<T>(name, count);
-  MDA_HIP(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  h2d(c, d, src, count * sizeof(T));
   return d;
 }
 
@@ -198,8 +215,15 @@ struct letkf_solve_args {
   double* T;
   int* sweeps;
   int* status;
+  // tridiagonal eigensolver workspace (symmetric square root, k <= 64)
+  double* de;    // [ncol][2k]
+  double* lam;   // [ncol][k]
+  int* flag;     // [ncol] columns re-solved by the Jacobi fallback
+  int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
 };
 void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
+void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
+                      int* flag);
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
                             const double* T);
 void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,

@@ -51,12 +51,13 @@ def smooth_fields(rng, m, nlev, nlat, nlon, nmodes=12):
     return out.reshape(m, -1)
 
 
-def latlon_case(nlon=64, nlat=32, nlev=4, k=20, p=3000, sigma=0.5, seed=11, cluster=False):
+def latlon_case(nlon=64, nlat=32, nlev=4, k=20, p=3000, sigma=0.5, seed=11, cluster=False,
+                nmodes=12):
     """Scaled-down global LETKF case: background = smooth random fields,
     truth = one more draw, obs area-uniform with trilinear H."""
     from metada_amd import ObsOperator
     rng = np.random.default_rng(seed)
-    F = smooth_fields(rng, k + 1, nlev, nlat, nlon)
+    F = smooth_fields(rng, k + 1, nlev, nlat, nlon, nmodes)
     truth, X = F[0], F[1:]
     if cluster:
         c_lat = np.degrees(np.arcsin(rng.uniform(-1, 1, 5)))

@@ -233,3 +233,78 @@ def test_error_paths(mda, ctx):
     with pytest.raises(MdaError, match="MDA_E_ARG"):
         mda.LETKF(mda.Ensemble(X.copy(), 10, 1, 1), obs, H,
                   mda.Config({"inflation": 1.0, "localization_radius": -1.0}), ctx=ctx).Analyse()
+
+
+@pytest.mark.parametrize("k", [64, 20])
+def test_letkf_tridiagonal_eigensolver_matches_jacobi(mda, ctx, orc, k):
+    """The default symmetric square root (tridiagonal + twisted factorisation)
+    against the parallel Jacobi path and the oracle on the same inputs.  The
+    fields carry many modes so Y'^T R^-1 Y' has a well-separated spectrum, as
+    Gaussian random fields do."""
+    case = latlon_case(k=k, p=6000, seed=40 + k, nmodes=96)
+    out = {}
+    for name, meth in (("auto", mda._lib.EIG_AUTO), ("jacobi", mda._lib.EIG_JACOBI)):
+        ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+        obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                              coord=mda._lib.COORD_GEOGRAPHIC)
+        geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+        f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                       "localization_radius": 1800.0}),
+                      geometry=geom, ctx=ctx, eig_method=meth)
+        diag = f.Analyse()
+        out[name] = (ens.data.copy(), diag)
+    # the odd column with a close, significant eigenpair goes to Jacobi
+    assert out["auto"][1]["n_fallback_columns"] <= 0.01 * case["nlon"] * case["nlat"]
+    assert out["jacobi"][1]["max_jacobi_sweeps"] > 0
+    assert rel_err(out["auto"][0], out["jacobi"][0]) < 1e-11
+    prob = orc.Problem(k, case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert rel_err(out["auto"][0], Xo) < TOL
+
+
+def test_letkf_rank_deficient_gram_needs_no_fallback(mda, ctx, orc):
+    """Few local observations make Y'^T R^-1 Y' rank deficient, so A has the
+    eigenvalue (k-1)/rho with high multiplicity; W is formed around it, so the
+    tridiagonal path handles these columns itself."""
+    rng = np.random.default_rng(12)
+    nx, ny, k, p = 10, 8, 16, 12
+    X = rng.standard_normal((k, nx * ny))
+    ox, oy = rng.uniform(0, nx - 1, p), rng.uniform(0, ny - 1, p)
+    obs = mda.Observation(rng.standard_normal(p), np.full(p, 0.5), ox, oy, np.zeros(p))
+    H = mda.ObsOperator.identity_idw4(obs, nx, ny, 1)
+    ens = mda.Ensemble(X.copy(), nx, ny, 1)
+    f = mda.LETKF(ens, obs, H, mda.Config({"inflation": 1.1, "localization_radius": 2.5}),
+                  ctx=ctx)
+    diag = f.Analyse()
+    assert diag["n_fallback_columns"] == 0
+    prob = orc.Problem(k, nx, ny, 1, orc.COORD_GRID, np.tile(np.arange(nx, dtype=float), ny),
+                       np.repeat(np.arange(ny, dtype=float), nx), ox, oy, obs.values,
+                       obs.variances, f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, X, H.apply_host(X))
+    assert rel_err(ens.data, Xo) < TOL
+
+
+def test_letkf_repeated_eigenvalue_falls_back_to_jacobi(mda, ctx, orc):
+    """Two co-located observations whose anomaly rows are orthogonal with equal
+    norms give A a double eigenvalue away from (k-1)/rho: the twisted-
+    factorisation vectors coincide, the orthogonality check flags the column
+    and the Jacobi kernel re-solves it."""
+    k = 8
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((k, 3))
+    hx = np.zeros((k, 2))
+    hx[:, 0] = [1, -1, 0, 0, 0, 0, 0, 0]
+    hx[:, 1] = [0, 0, 1, -1, 0, 0, 0, 0]
+    yo = np.array([0.3, -0.2])
+    obs = mda.Observation(yo, np.full(2, 0.25), np.zeros(2), np.zeros(2), np.zeros(2))
+    ens = mda.Ensemble(X.copy(), 1, 1, 3)
+    f = mda.LETKF(ens, obs, mda.ObsOperator(hx=hx),
+                  mda.Config({"inflation": 1.05, "localization_radius": 1.0}), ctx=ctx)
+    diag = f.Analyse()
+    assert diag["n_fallback_columns"] == 1
+    prob = orc.Problem(k, 1, 1, 3, orc.COORD_GRID, [0.0], [0.0], np.zeros(2), np.zeros(2), yo,
+                       np.full(2, 0.25), f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, X, hx)
+    assert rel_err(ens.data, Xo) < TOL
