@@ -532,7 +532,7 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     launch_zero(s, stats, sizeof(int64_t) * 9);
     launch_reduce_int(s, nloc, ncol, stats, nullptr);
     launch_reduce_int(s, sweeps, ncol, stats + 3, nullptr);
-    const bool eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && k <= 64 &&
+    const bool eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(k) &&
                           prm->eig_method == MDA_EIG_AUTO;
     if (eig_path) launch_reduce_int(s, sa.flag, ncol, stats + 6, nullptr);
     int64_t hs[9] = {0};

@@ -23,13 +23,19 @@ namespace mda {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int EIG_N = 64;  // columns handled by this path have k <= 64
+constexpr int EIG_N = 64;  // largest ensemble size handled by this path
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Compiler-only memory barrier (emits no instruction).  Placed every few
+// unrolled iterations it makes the compiler re-read broadcast LDS values
+// instead of keeping all 2N of d, e (or a whole reflector row) live in
+// registers across loops, which would otherwise cost occupancy and spills.
+__device__ inline void lds_reload_point() { asm volatile("" ::: "memory"); }
 
 __device__ inline double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -41,66 +47,81 @@ __device__ inline double wave_sum(double v) {
 //   x = A[j+1:, j],  alpha = -sign(x_0) ||x||,  u = x - alpha e_1,
 //   tau = 2 / u^T u,  p = tau A' u,  w = p - (tau/2)(p^T u) u,
 //   A' <- A' - u w^T - w u^T   (A' = A[j+1:, j+1:]),  e_j = alpha.
+// Lane r keeps row r of A in registers (static column index); column j is
+// read as row j (A is symmetric) through one LDS row.  The reflector j is
+// stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j, and tau_j in
+// U[j][0] (never a reflector position).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_trd(int n, int64_t ncol, const double* __restrict__ C,
+template <int N>
+__global__ __launch_bounds__(64) void k_trd(int64_t ncol, const double* __restrict__ C,
                                             const int* __restrict__ nloc, double rho,
                                             const double* __restrict__ rho_col, int prior,
                                             double* __restrict__ de, double* __restrict__ Tslot) {
-  __shared__ double A[EIG_N * (EIG_N + 1)];
-  __shared__ double ub[EIG_N], wb[EIG_N];
-  const int lane = threadIdx.x, ld = EIG_N + 1;
+  constexpr int n = N;
+  __shared__ double row[EIG_N], ub[EIG_N], wb[EIG_N];
+  const int lane = threadIdx.x;
   for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
     if (nloc[g] == 0) continue;
     const double r = rho_col ? rho_col[g] : rho;
     const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
     const double* Cg = C + (size_t)g * n * n;
-    for (int idx = lane; idx < n * n; idx += 64) {
-      int i = idx / n, c = idx % n;
-      A[i * ld + c] = Cg[idx] + (i == c ? sigma : 0.0);
+    const int rl = lane < n ? lane : n - 1;  // clamped row: loads stay in bounds
+    double ar[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      double v = Cg[(size_t)rl * n + c];
+      ar[c] = lane < n ? v + (lane == c ? sigma : 0.0) : 0.0;
     }
-    wave_lds_sync();
     double* U = Tslot + (size_t)g * n * n;
     double* dg = de + (size_t)g * 2 * n;
     for (int j = 0; j < n - 2; ++j) {
-      const int m = n - j - 1, o = j + 1;
-      double xi = lane < m ? A[(o + lane) * ld + j] : 0.0;
-      double nrm2 = wave_sum(xi * xi);
-      double x0 = __shfl(xi, 0);
+      if (lane == j) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) row[c] = ar[c];
+      }
+      wave_lds_sync();
+      const double xi = (lane > j && lane < n) ? row[lane] : 0.0;  // A[lane][j] = A[j][lane]
+      const double nrm2 = wave_sum(xi * xi);
+      const double x0 = row[j + 1];
       double alpha = 0.0, tau = 0.0, ui = 0.0;
       if (nrm2 > 0.0) {
         alpha = (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2);
-        ui = xi - (lane == 0 ? alpha : 0.0);
-        double unrm2 = nrm2 - 2.0 * alpha * x0 + alpha * alpha;
-        tau = 2.0 / unrm2;
+        ui = xi - (lane == j + 1 ? alpha : 0.0);
+        tau = 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha);
       }
-      if (lane == 0) { dg[j] = A[j * ld + j]; dg[n + j] = alpha; }
-      if (lane < m) U[(size_t)j * n + lane] = ui;
-      if (lane == 0) U[(size_t)j * n + n - 1] = tau;
-      if (tau == 0.0) continue;  // x = 0: H = I, A' unchanged
+      if (lane == 0) { dg[j] = row[j]; dg[n + j] = alpha; }
+      if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
+      if (tau == 0.0) { wave_lds_sync(); continue; }  // x = 0: H = I
       ub[lane] = ui;
       wave_lds_sync();
-      double pi = 0.0;
-      if (lane < m) {
-        const double* row = A + (o + lane) * ld + o;
-        for (int c = 0; c < m; ++c) pi += row[c] * ub[c];
-        pi *= tau;
-      }
-      double K = 0.5 * tau * wave_sum(pi * ui);
-      double wi = pi - K * ui;
+      double p = 0.0;
+#pragma unroll
+      for (int c = 0; c < N; ++c) p += ar[c] * ub[c];
+      p *= tau;
+      const double K = 0.5 * tau * wave_sum(p * ui);
+      const double wi = (lane > j && lane < n) ? p - K * ui : 0.0;
       wb[lane] = wi;
       wave_lds_sync();
-      if (lane < m) {
-        for (int rr = 0; rr < m; ++rr) {
-          double* a = A + (o + rr) * ld + o + lane;
-          *a -= ub[rr] * wi + wb[rr] * ui;
-        }
-      }
+#pragma unroll
+      for (int c = 0; c < N; ++c) ar[c] -= ui * wb[c] + wi * ub[c];
       wave_lds_sync();
     }
+    // last 2x2 block: rows n-2 and n-1 through the LDS row
+    if (lane == n - 2) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) row[c] = ar[c];
+    }
+    wave_lds_sync();
+    if (lane == 0) dg[n - 2] = row[n - 2];
+    wave_lds_sync();
+    if (lane == n - 1) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) row[c] = ar[c];
+    }
+    wave_lds_sync();
     if (lane == 0) {
-      dg[n - 2] = A[(n - 2) * ld + n - 2];
-      dg[n - 1] = A[(n - 1) * ld + n - 1];
-      dg[n + n - 2] = A[(n - 1) * ld + n - 2];
+      dg[n - 1] = row[n - 1];
+      dg[n + n - 2] = row[n - 2];
       dg[n + n - 1] = 0.0;
     }
     wave_lds_sync();
@@ -112,12 +133,13 @@ __global__ __launch_bounds__(64) void k_trd(int n, int64_t ncol, const double* _
 // Wilkinson-type shift (Golub & Van Loan, sec. 8.3, QL orientation), one lane
 // per column, arrays in LDS as [index][lane].  Ascending output.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_trd_eigvals(int n, int64_t ncol,
-                                                    const int* __restrict__ nloc,
+template <int N>
+__global__ __launch_bounds__(64) void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc,
                                                     const double* __restrict__ de,
                                                     double* __restrict__ lam,
                                                     int* __restrict__ flag) {
-  __shared__ double sd[EIG_N * 64], se[EIG_N * 64];
+  constexpr int n = N;
+  __shared__ double sd[N * 64], se[N * 64];
   const int lane = threadIdx.x;
   const int64_t g = (int64_t)blockIdx.x * 64 + lane;
   const bool active = g < ncol && nloc[g] != 0;
@@ -197,16 +219,23 @@ __device__ inline double safe_pivot(double v, double tiny) {
   return fabs(v) < tiny ? (v < 0.0 ? -tiny : tiny) : v;
 }
 
+template <int N>
 __global__ __launch_bounds__(64) void k_trd_vectors(
-    int n, int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+    int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
     const double* __restrict__ rho_col, int prior, int combine, double* __restrict__ T,
     int* __restrict__ flag) {
-  __shared__ double Z[EIG_N * (EIG_N + 1)];
-  __shared__ double sd[EIG_N], se[EIG_N], sl[EIG_N], sy[EIG_N], sb[EIG_N], swt[EIG_N];
-  constexpr int BAND = 8;                 // near pairs (j, j+1..j+BAND) of the sorted spectrum
-  __shared__ double dots[EIG_N][BAND];
-  const int lane = threadIdx.x, ld = EIG_N + 1;
+  constexpr int n = N;
+  constexpr int NP = (N + 15) / 16 * 16;  // MFMA tile padding
+  constexpr int ld = NP + 1;              // Z[i * ld + j]: component i of vector j
+  constexpr int ZS = NP * ld > N * 64 ? NP * ld : N * 64;
+  constexpr int BAND = 8;  // near pairs (j, j+1..j+BAND) of the sorted spectrum
+  // Z doubles as the per-lane U_i store [i * 64 + lane] of the factorisation
+  // and as the staging area of the Householder rows for the back-transform.
+  __shared__ double Z[ZS];
+  __shared__ double sd[64], se[64], sl[64], sy[64], sb[64], swt[64];
+  __shared__ double dots[64][BAND];
+  const int lane = threadIdx.x;
   for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
     if (nloc[g] == 0 || flag[g]) continue;  // empty columns / Jacobi fallback columns
     const double r = rho_col ? rho_col[g] : rho;
@@ -218,64 +247,60 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     sb[lane] = lane < n ? bvec[(size_t)g * n + lane] : 0.0;
     wave_lds_sync();
     double tnorm = 0.0;
-    for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if ((i & 7) == 0) lds_reload_point();
+      tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+    }
     const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
     const double l = sl[lane < n ? lane : 0];
-    // ---- twisted factorisation of T - l I ----
-    double Lv[EIG_N], Uv[EIG_N];
-    double dp = sd[0] - l;
+    // ---- eigenvector z of T for lambda_lane, in registers ----
+    // L_i built in place in z[], U_i parked in LDS, then
+    // z_kt = 1, z_i = -L_i z_{i+1} (i < kt), z_{i+1} = -U_i z_i (i >= kt)
+    double z[N];
+    double* Uv = Z;
+    {
+      double dp = sd[0] - l;
 #pragma unroll
-    for (int i = 0; i < EIG_N - 1; ++i) {
-      if (i < n - 1) {
-        double piv = safe_pivot(dp, tiny);
-        Lv[i] = se[i] / piv;
-        dp = sd[i + 1] - l - Lv[i] * se[i];
+      for (int i = 0; i < N - 1; ++i) {
+        if ((i & 7) == 0) lds_reload_point();
+        z[i] = se[i] / safe_pivot(dp, tiny);  // L_i
+        dp = sd[i + 1] - l - z[i] * se[i];
       }
-    }
-    double gbest = fabs(dp);  // gamma_{n-1} = D+_{n-1}
-    int kt = n - 1;
-    double dm = sd[n - 1] - l;
+      z[N - 1] = 0.0;
+      double gbest = fabs(dp);  // gamma_{n-1} = D+_{n-1}
+      int kt = N - 1;
+      double dm = sd[N - 1] - l;
 #pragma unroll
-    for (int i = EIG_N - 2; i >= 0; --i) {
-      if (i < n - 1) {
-        double piv = safe_pivot(dm, tiny);
-        Uv[i] = se[i] / piv;
-        dm = sd[i] - l - Uv[i] * se[i];
-        double dpi = sd[i] - l;
-        if (i > 0) dpi -= Lv[i > 0 ? i - 1 : 0] * se[i > 0 ? i - 1 : 0];
-        double gam = fabs(dpi + dm - (sd[i] - l));
+      for (int i = N - 2; i >= 0; --i) {
+        if ((i & 7) == 6) lds_reload_point();
+        const double u = se[i] / safe_pivot(dm, tiny);
+        Uv[i * 64 + lane] = u;
+        dm = sd[i] - l - u * se[i];
+        const double dpi = sd[i] - l - (i > 0 ? z[i > 0 ? i - 1 : 0] * se[i > 0 ? i - 1 : 0] : 0.0);
+        const double gam = fabs(dpi + dm - (sd[i] - l));
         if (gam < gbest) { gbest = gam; kt = i; }
       }
-    }
-    // z_kt = 1; downward z_i = -L_i z_{i+1} kept in Lv[i]; upward z_{i+1} = -U_i z_i in Uv[i]
-    double zc = 1.0;
+      double zc = 1.0;
 #pragma unroll
-    for (int i = EIG_N - 2; i >= 0; --i) {
-      if (i < kt) { zc = -Lv[i] * zc; Lv[i] = zc; }
-    }
-    zc = 1.0;
+      for (int i = N - 2; i >= 0; --i)
+        if (i < kt) { zc = -z[i] * zc; z[i] = zc; }
+      double nrm = 0.0;
+      zc = 1.0;
 #pragma unroll
-    for (int i = 0; i < EIG_N - 1; ++i) {
-      if (i >= kt && i < n - 1) { zc = -Uv[i] * zc; Uv[i] = zc; }
-    }
-    double nrm = 0.0;
-#pragma unroll
-    for (int i = 0; i < EIG_N; ++i) {
-      if (i < n) {
-        double z = i < kt ? Lv[i < EIG_N - 1 ? i : 0] : (i == kt ? 1.0 : Uv[i > 0 ? i - 1 : 0]);
-        nrm += z * z;
+      for (int i = 0; i < N; ++i) {
+        if ((i & 7) == 0) lds_reload_point();
+        const double cand = i > 0 ? -Uv[(i > 0 ? i - 1 : 0) * 64 + lane] * zc : 0.0;
+        const double v = i < kt ? z[i] : (i == kt ? 1.0 : cand);
+        if (i >= kt) zc = v;
+        z[i] = lane < n ? v : 0.0;
+        nrm += z[i] * z[i];
       }
-    }
-    const double inv = 1.0 / sqrt(nrm);
-    // column `lane` of S into LDS (rows >= n and columns >= n stay zero)
+      const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
 #pragma unroll
-    for (int i = 0; i < EIG_N; ++i) {
-      double z = 0.0;
-      if (i < n && lane < n)
-        z = (i < kt ? Lv[i < EIG_N - 1 ? i : 0] : (i == kt ? 1.0 : Uv[i > 0 ? i - 1 : 0])) * inv;
-      Z[i * ld + lane] = z;
+      for (int i = 0; i < N; ++i) z[i] *= inv;
     }
-    wave_lds_sync();
+    wave_lds_sync();  // U_i reads done before Z is written
     // ---- eigenpairs at sigma = (k-1)/rho (prior) or k-1 (posterior) ----
     // W and wbar are formed around sigma:
     //   W    = c [ f(sigma) I - sum_j g_j z_j z_j^T ],  g_j = sigma^-1/2 - lambda_j^-1/2
@@ -290,33 +315,44 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     const double wj = skip ? 0.0 : 1.0 - sigma / l;
     sy[lane] = skip ? 1.0 : 0.0;  // skip flags, shared
     swt[lane] = wj;
-    wave_lds_sync();
-    // ---- checks: eigen-residual of T; close pairs re-orthogonalised ----
     int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
     if (!skip) {
       double res = 0.0;
-      for (int i = 0; i < n; ++i) {
-        double v = (sd[i] - l) * Z[i * ld + lane];
-        if (i > 0) v += se[i - 1] * Z[(i - 1) * ld + lane];
-        if (i < n - 1) v += se[i] * Z[(i + 1) * ld + lane];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        if ((i & 7) == 0) lds_reload_point();
+        double v = (sd[i] - l) * z[i];
+        if (i > 0) v += se[i > 0 ? i - 1 : 0] * z[i > 0 ? i - 1 : 0];
+        if (i < N - 1) v += se[i] * z[i < N - 1 ? i + 1 : 0];
         res = fmax(res, fabs(v));
       }
       if (!(res <= 1e-12 * tnorm)) why |= 2;
       if (!(l >= sigma)) why |= 8;
     }
+    if (lane < NP) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i];
+#pragma unroll
+      for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
+    }
+    wave_lds_sync();
     // Close eigenpairs from twisted factorisations are accurate except for a
     // small component of each along the other; a symmetric (Lowdin) step
     // z_j -= 1/2 sum_q (z_j.z_q) z_q removes the non-orthogonality to second
     // order and leaves only a rotation inside the pair, which changes W by
     // (g_j - g_q) times that rotation: negligible for close eigenvalues.
+    bool any_near = false;
     for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
       for (int q = 0; q < BAND; ++q) dots[lane][q] = 0.0;
+      bool near = false;
       if (!skip) {
         for (int q = lane + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
           if (sy[q] != 0.0) continue;
+          near = true;
           double dot = 0.0;
-          for (int i = 0; i < n; ++i) dot += Z[i * ld + lane] * Z[i * ld + q];
+#pragma unroll
+          for (int i = 0; i < N; ++i) dot += z[i] * Z[i * ld + q];
           if (q - lane > BAND) {
             if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;  // cluster wider than the band
           } else {
@@ -326,24 +362,23 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
         }
       }
       wave_lds_sync();
-      if (pass == 1) break;
-      double znew[EIG_N];
-#pragma unroll
-      for (int i = 0; i < EIG_N; ++i) znew[i] = Z[i * ld + lane];
+      if (pass == 0) any_near = __ballot(near) != 0ull;
+      if (pass == 1 || !any_near) break;
       if (!skip) {
-        for (int q = 0; q < n; ++q) {
-          double c = 0.0;
-          if (q > lane && q - lane <= BAND) c = dots[lane][q - lane - 1];
-          if (q < lane && lane - q <= BAND) c = dots[q][lane - q - 1];
+        for (int q = lane - BAND; q <= lane + BAND; ++q) {
+          if (q < 0 || q >= n || q == lane) continue;
+          const double c = q > lane ? dots[lane][q - lane - 1] : dots[q][lane - q - 1];
           if (c != 0.0) {
 #pragma unroll
-            for (int i = 0; i < EIG_N; ++i) znew[i] -= 0.5 * c * Z[i * ld + q];
+            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Z[i * ld + q];
           }
         }
       }
       wave_lds_sync();
+      if (lane < NP) {
 #pragma unroll
-      for (int i = 0; i < EIG_N; ++i) Z[i * ld + lane] = znew[i];
+        for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i];
+      }
       wave_lds_sync();
     }
     for (int off = 32; off > 0; off >>= 1) why |= __shfl_xor(why, off);
@@ -352,55 +387,91 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
       wave_lds_sync();
       continue;
     }
-    wave_lds_sync();
-    // ---- Z = Q S: reflectors in reverse order (u_j, tau_j parked in Tg) ----
-    for (int j = n - 3; j >= 0; --j) {
-      const int m = n - j - 1, o = j + 1;
-      const double* u = Tg + (size_t)j * n;
-      const double tau = u[n - 1];
-      if (tau == 0.0) continue;
-      double dot = 0.0;
-      for (int i = 0; i < m; ++i) dot += u[i] * Z[(o + i) * ld + lane];
-      dot *= tau;
-      for (int i = 0; i < m; ++i) Z[(o + i) * ld + lane] -= dot * u[i];
-    }
-    wave_lds_sync();
-    // ---- wbar = s [ b/sigma - Z diag(1/sigma - 1/lambda) Z^T b ] ----
-    const double sc = pri ? 1.0 : r;
+    // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
+    // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
+    // r = 0) are staged in LDS with coalesced loads and read as broadcasts.
+    // Reflectors are taken in chunks of 16 so the static inner range starts
+    // at the chunk's first non-zero position.
     {
-      double y = 0.0;
-      if (!skip) {
-        for (int i = 0; i < n; ++i) y += Z[i * ld + lane] * sb[i];
-        y *= (1.0 / sigma - 1.0 / l);
+      double* Us = Z;
+      constexpr int NN = N * N;
+#pragma unroll
+      for (int t = 0; t < (NN + 63) / 64; ++t) {
+        const int idx = t * 64 + lane;
+        if (idx < NN) Us[idx] = Tg[idx];
       }
-      sy[lane] = y;
+      wave_lds_sync();
+      constexpr int NC = (N - 2 + 15) / 16;
+#pragma unroll
+      for (int c = NC - 1; c >= 0; --c) {
+        const int jlo = 16 * c;
+        const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
+#pragma unroll 1
+        for (int j = jhi; j >= jlo; --j) {
+          const double* u = Us + j * N;
+          const double tau = u[0];
+          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+          for (int i = jlo + 1; i < N; ++i) {
+            if ((i & 15) == 0) lds_reload_point();
+            const double t = u[i] * z[i];
+            switch (i & 3) {
+              case 0: p0 += t; break;
+              case 1: p1 += t; break;
+              case 2: p2 += t; break;
+              default: p3 += t; break;
+            }
+          }
+          const double dot = tau * ((p0 + p1) + (p2 + p3));
+#pragma unroll
+          for (int i = jlo + 1; i < N; ++i) {
+            if ((i & 15) == 0 || i == jlo + 1) lds_reload_point();
+            z[i] -= dot * u[i];
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+    // ---- U_s = Z diag(sqrt(g)) and y' for wbar ----
+    const double sc = pri ? 1.0 : r;
+    const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
+    double yb = 0.0;
+    if (!skip) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) yb += z[i] * sb[i];
+      yb *= (1.0 / sigma - 1.0 / l);
+    }
+    sy[lane] = f > 0.0 ? yb / f : 0.0;  // Z y = U_s (y / f)
+    if (lane < NP) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i] * f;
+#pragma unroll
+      for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
     }
     wave_lds_sync();
+    // wbar_r = s [ b_r / sigma - sum_j U_s[r][j] y'_j ]
     double wbar = 0.0;
     if (lane < n) {
       double acc = 0.0;
-      for (int e2 = 0; e2 < n; ++e2) acc += Z[lane * ld + e2] * sy[e2];
+#pragma unroll
+      for (int e2 = 0; e2 < N; ++e2) acc += Z[lane * ld + e2] * sy[e2];
       wbar = sc * (sb[lane] / sigma - acc);
     }
-    wave_lds_sync();
-    // ---- U_s = Z diag(sqrt(g)) (scale column `lane`) ----
-    {
-      double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
-      for (int i = 0; i < n; ++i) Z[i * ld + lane] *= f;
-    }
-    wave_lds_sync();
     const double coef = sqrt((double)(n - 1)) * sqrt(sc);
     const double fs = 1.0 / sqrt(sigma);
     if (combine == MDA_COMBINE_ELEMENTWISE) {
-      // t_i = wbar_i + coef sum_a U_s[i][a] (sum_j U_s[j][a])
+      // t_i = wbar_i + coef (f(sigma) - sum_a U_s[i][a] (sum_j U_s[j][a]))
       double cs = 0.0;
-      for (int j = 0; j < n; ++j) cs += Z[j * ld + lane];
+#pragma unroll
+      for (int i = 0; i < N; ++i) cs += z[i] * f;
+      wave_lds_sync();
       sy[lane] = cs;
       wave_lds_sync();
       double t = wbar;
       if (lane < n) {
         double acc = 0.0;
-        for (int a = 0; a < n; ++a) acc += Z[lane * ld + a] * sy[a];
+#pragma unroll
+        for (int a = 0; a < N; ++a) acc += Z[lane * ld + a] * sy[a];
         t += coef * (fs - acc);
       }
       for (int idx = lane; idx < n * n; idx += 64) Tg[idx] = 0.0;
@@ -408,24 +479,28 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
       wave_lds_sync();
       continue;
     }
+    wave_lds_sync();
     sy[lane] = wbar;
     wave_lds_sync();
-    // ---- W = coef U_s U_s^T on FP64 MFMA (lower 16x16 tiles, mirrored) ----
-    const int nt = (n + 15) / 16;
+    // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA (lower tiles, mirrored) ----
+    constexpr int nt = NP / 16;
     const int i16 = lane & 15, kk = lane >> 4;
+#pragma unroll 1
     for (int ti = 0; ti < nt; ++ti) {
+#pragma unroll 1
       for (int tj = 0; tj <= ti; ++tj) {
         d4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int s4 = 0; s4 < 16; ++s4) {
-          double av = Z[(16 * ti + i16) * ld + 4 * s4 + kk];
-          double bv = Z[(16 * tj + i16) * ld + 4 * s4 + kk];
+#pragma unroll
+        for (int s4 = 0; s4 < NP / 4; ++s4) {
+          const double av = Z[(16 * ti + i16) * ld + 4 * s4 + kk];
+          const double bv = Z[(16 * tj + i16) * ld + 4 * s4 + kk];
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
+          const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
           if (row < n && col < n) {
-            double w = coef * ((row == col ? fs : 0.0) - acc[q]);
+            const double w = coef * ((row == col ? fs : 0.0) - acc[q]);
             Tg[(size_t)row * n + col] = sy[row] + w;
             if (ti != tj) Tg[(size_t)col * n + row] = sy[col] + w;
           }
@@ -437,22 +512,42 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
 }
 
 // ---------------------------------------------------------------------------
-void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
-                      int* flag) {
-  const int n = a.k;
-  MDA_REQUIRE(n >= 2 && n <= EIG_N, "tridiagonal eigensolver path supports k <= 64");
+template <int N>
+static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
+                         int* flag) {
   const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
   launch_zero(s, flag, sizeof(int) * a.ncol);
-  unsigned nb = (unsigned)(a.ncol < 131072 ? a.ncol : 131072);
-  hipLaunchKernelGGL(k_trd, dim3(nb), dim3(64), 0, s, n, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
+  const unsigned nb = (unsigned)(a.ncol < 131072 ? a.ncol : 131072);
+  hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
                      prior, de, a.T);
   MDA_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_trd_eigvals, dim3((unsigned)((a.ncol + 63) / 64)), dim3(64), 0, s, n,
+  hipLaunchKernelGGL(k_trd_eigvals<N>, dim3((unsigned)((a.ncol + 63) / 64)), dim3(64), 0, s,
                      a.ncol, a.nloc, de, lam, flag);
   MDA_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_trd_vectors, dim3(nb), dim3(64), 0, s, n, a.ncol, a.nloc, de, lam, a.b,
+  hipLaunchKernelGGL(k_trd_vectors<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.nloc, de, lam, a.b,
                      a.rho, a.rho_col, prior, a.combine, a.T, flag);
   MDA_LAUNCH_CHECK();
+}
+
+bool letkf_eig_supported(int k) {
+  switch (k) {
+    case 8: case 16: case 20: case 32: case 40: case 48: case 64: return true;
+    default: return false;
+  }
+}
+
+void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
+                      int* flag) {
+  switch (a.k) {
+    case 8: launch_eig_n<8>(s, a, de, lam, flag); break;
+    case 16: launch_eig_n<16>(s, a, de, lam, flag); break;
+    case 20: launch_eig_n<20>(s, a, de, lam, flag); break;
+    case 32: launch_eig_n<32>(s, a, de, lam, flag); break;
+    case 40: launch_eig_n<40>(s, a, de, lam, flag); break;
+    case 48: launch_eig_n<48>(s, a, de, lam, flag); break;
+    case 64: launch_eig_n<64>(s, a, de, lam, flag); break;
+    default: throw mda_error(MDA_E_UNSUPPORTED, "tridiagonal eigensolver: no kernel for this k");
+  }
 }
 
 }  // namespace mda

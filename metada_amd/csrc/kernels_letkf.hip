@@ -535,7 +535,7 @@ void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a) {
   double* gscratch = nullptr;
   if (!use_lds) throw mda_error(MDA_E_UNSUPPORTED, "LETKF solve for k > 96 not in this build");
   const int* only = nullptr;
-  if (a.sqrt_form == MDA_SQRT_SYMMETRIC && a.k <= 64 && a.method == 0) {
+  if (a.sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(a.k) && a.method == 0) {
     // tridiagonal eigensolver; flagged columns fall back to Jacobi below
     launch_letkf_eig(s, a, a.de, a.lam, a.flag);
     only = a.flag;

@@ -222,6 +222,8 @@ struct letkf_solve_args {
   int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
 };
 void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
+// ensemble sizes with a compiled tridiagonal eigensolver (others use Jacobi)
+bool letkf_eig_supported(int k);
 void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                       int* flag);
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
