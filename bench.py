@@ -28,9 +28,11 @@ PEAK = {"hbm_GBs": 8000.0, "fp64_TFs": 78.6, "fp32_TFs": 157.3, "bf16_TFs": 2500
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="skip the FP64 Jacobi (plain) path")
     ap.add_argument("--nlon", type=int, default=1024)
     ap.add_argument("--nlat", type=int, default=512)
     ap.add_argument("--nlev", type=int, default=40)
@@ -62,7 +64,6 @@ def main():
     from metada_amd import synthetic
     from metada_amd import _lib as L
 
-    prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}[a.precision]
     dev = "cuda:0"
     t0 = time.time()
     W = synthetic.latlon_workload(a.nlon, a.nlat, a.nlev, a.k, a.p, device=dev, progress=log)
@@ -86,7 +87,6 @@ def main():
     yo = W["obs"].values.cpu().numpy()
     var = W["obs"].variances.cpu().numpy()
     rho32 = float(np.float32(a.rho))
-    parity = None
     cpu = None
     T_oracle = None
     try:
@@ -119,63 +119,145 @@ def main():
         log(f"oracle unavailable: {e!r}")
 
     # ---- GPU analyses ----
+    # Every analysis starts from the same background X_b (restored outside the
+    # timed region), so the paths are timed on, and compared over, one input.
     ctx = mda.Context(0)
-    ens = mda.Ensemble(W["X"], a.nlon, a.nlat, nlev)
+    Xb = W["X"].clone()
     cfg = mda.Config({"inflation": a.rho, "localization_radius": a.radius})
-    f = mda.LETKF(ens, W["obs"], W["H"], cfg, geometry=W["geom"], ctx=ctx, precision=prec)
-    for w in range(max(1, a.warmup)):
-        d0 = f.Analyse()
-        if w == 0 and T_oracle is not None:
-            Tg = f.transforms(cols)
-            err = float(np.max(np.abs(Tg - T_oracle)) / np.max(np.abs(T_oracle)))
-            parity = {"oracle": "standard LETKF (C, FP64)", "columns": len(cols),
-                      "max_rel_err_T": err}
-            log(f"parity on {len(cols)} sampled columns: max rel err of T = {err:.3e}")
-        if d0["n_fallback_columns"]:
-            fl = f.column_flags()
-            log("fallback reasons: " + ", ".join(f"{nm}={int(((fl & bit) != 0).sum())}" for bit, nm in
-                ((1, "ql"), (2, "residual"), (4, "orthogonality"), (8, "below_sigma"))))
-        log(f"warmup {w}: {d0['ms_total']:.1f} ms  (gram {d0['ms_gram']:.1f}, solve "
-            f"{d0['ms_solve']:.1f}, transform {d0['ms_transform']:.1f}, obs {d0['ms_obs']:.1f},"
-            f" bucket {d0['ms_bucket']:.1f}; fallback columns {d0['n_fallback_columns']})")
+    ens = mda.Ensemble(W["X"], a.nlon, a.nlat, nlev)
+
+    def make(prec_code, eig):
+        return mda.LETKF(ens, W["obs"], W["H"], cfg, geometry=W["geom"], ctx=ctx,
+                         precision=prec_code, eig_method=eig)
+
+    def run(name, f, warmup, steps, check_parity=False):
+        times, diags = [], []
+        for it in range(warmup + steps):
+            W["X"].copy_(Xb)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            d = f.Analyse()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t1) * 1000.0
+            if it == 0 and check_parity and T_oracle is not None:
+                Tg = f.transforms(cols)
+                err = float(np.max(np.abs(Tg - T_oracle)) / np.max(np.abs(T_oracle)))
+                nonlocal_parity[name] = err
+                log(f"{name}: max rel err of T vs the oracle on {len(cols)} columns = {err:.3e}")
+            if it == 0 and d["n_fallback_columns"]:
+                fl = f.column_flags()
+                log(f"{name}: fallback reasons: " + ", ".join(
+                    f"{nm}={int(((fl & bit) != 0).sum())}" for bit, nm in
+                    ((1, "ql"), (2, "residual"), (4, "orthogonality"), (8, "below_sigma"))))
+            tag = "warmup" if it < warmup else "step"
+            log(f"{name} {tag} {it if it < warmup else it - warmup}: {dt:.1f} ms (lib "
+                f"{d['ms_total']:.1f}: obs {d['ms_obs']:.1f}, bucket {d['ms_bucket']:.1f}, gram "
+                f"{d['ms_gram']:.1f}, solve {d['ms_solve']:.1f} [trd {d['ms_eig_tridiag']:.1f}, "
+                f"vals {d['ms_eig_values']:.1f}, vecs {d['ms_eig_vectors']:.1f}, jacobi "
+                f"{d['ms_solve_jacobi']:.1f}], transform {d['ms_transform']:.1f}; fallback "
+                f"{d['n_fallback_columns']})")
+            if it >= warmup:
+                times.append(dt)
+                diags.append(d)
+        return np.array(times), diags
+
+    nonlocal_parity = {}
+    stage_keys = ("ms_obs", "ms_bucket", "ms_gram", "ms_solve", "ms_eig_tridiag",
+                  "ms_eig_values", "ms_eig_vectors", "ms_solve_jacobi", "ms_transform")
+
+    def summary(times, diags):
+        st = {key: float(np.mean([d[key] for d in diags])) for key in stage_keys}
+        return {"ms_median": float(np.median(times)), "ms_min": float(np.min(times)),
+                "ms_mean": float(np.mean(times)), "analyses_per_s": 1000.0 / float(np.median(times)),
+                "stages_ms": st, "fallback_columns": int(diags[-1]["n_fallback_columns"])}
+
+    # FP64, tridiagonal eigensolver (fastest FP64 path): reference output
+    f64 = make(L.PREC_FP64, L.EIG_AUTO)
+    t64, d64 = run("fp64", f64, 1, max(1, min(a.steps, 5)), check_parity=True)
+    Xd = W["X"].clone()
+    res = {"fp64": summary(t64, d64)}
+    # FP64 plain: FP64 contractions and the batched Jacobi eigensolver
+    if not a.skip_plain:
+        fpl = make(L.PREC_FP64, L.EIG_JACOBI)
+        tpl, dpl = run("fp64_jacobi", fpl, 1, max(1, min(a.steps, 3)), check_parity=True)
+        res["fp64_jacobi"] = summary(tpl, dpl)
+        res["fp64_jacobi"]["max_abs_diff_vs_fp64"] = float((W["X"] - Xd).abs().max())
+
+    def errors(Xm):
+        diff = Xm - Xd
+        inc = Xd - Xb
+        rms = lambda t: float(torch.sqrt(torch.mean(t * t)))
+        mean_m, mean_d, mean_b = Xm.mean(0), Xd.mean(0), Xb.mean(0)
+        return {"ensemble": {"rel_rms": rms(diff) / rms(Xd),
+                             "rel_max": float(diff.abs().max() / Xd.abs().max()),
+                             "rms_over_increment": rms(diff) / rms(inc)},
+                "mean": {"rel_rms": rms(mean_m - mean_d) / rms(mean_d),
+                         "rms_over_increment": rms(mean_m - mean_d) / rms(mean_d - mean_b)},
+                "perturbations": {"rel_rms": rms((Xm - mean_m) - (Xd - mean_d)) / rms(Xd - mean_d)}}
+
+    mixed = {}
+    for pname in ("fp32", "bf16x3"):
+        code = {"fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}[pname]
+        fm = make(code, L.EIG_AUTO)
+        is_value = pname == a.precision
+        tm, dm = run(pname, fm, a.warmup if is_value else 1,
+                     a.steps if is_value else max(1, min(a.steps, 3)))
+        mixed[pname] = summary(tm, dm)
+        mixed[pname]["error_vs_fp64"] = errors(W["X"])
+        log(f"{pname}: errors vs FP64 {json.dumps(mixed[pname]['error_vs_fp64'])}")
+    res.update(mixed)
+    if a.precision == "fp64":
+        f64b = make(L.PREC_FP64, L.EIG_AUTO)
+        tv, dv = run("fp64_value", f64b, a.warmup, a.steps)
+        vres = summary(tv, dv)
+        d_last = dv[-1]
+    else:
+        vres = res[a.precision]
+        d_last = None
     torch.cuda.synchronize()
-    stages = {s: 0.0 for s in ("ms_obs", "ms_bucket", "ms_gram", "ms_solve", "ms_transform")}
-    t_start = time.perf_counter()
-    for s in range(a.steps):
-        d = f.Analyse()
-        for key in stages:
-            stages[key] += d[key] / a.steps
-        log(f"step {s}: {d['ms_total']:.1f} ms")
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t_start) * 1000.0 / a.steps
+    ms = vres["ms_mean"]
     value = 1000.0 / ms
+    stages = vres["stages_ms"]
+    d = d_last or f64.diag
 
     # ---- roofline of the dominant kernel (algorithmic work / HIP-event time) ----
     tot_loc = d["total_local_obs"]
     nonempty = ncol - d["n_empty_columns"]
     gram_flop = 2.0 * tot_loc * k * k + 2.0 * tot_loc * k
-    solve_flop = nonempty * (9.0 * k ** 3 + 2.0 * k ** 3 + 4.0 * k * k)
     xform_bytes = 2.0 * n * k * 8
     peak_c = {"fp64": PEAK["fp64_TFs"], "fp32": PEAK["fp32_TFs"], "bf16x3": PEAK["bf16_TFs"]}
+    gram_kernel = "k_letkf_gram_f64" if a.precision == "fp64" else "k_letkf_gram_mixed"
+    # eigensolver kernels, priced at the nominal 9k^3 + 2k^3 + 4k^2 per column
+    # split as Householder 4/3 k^3, QL ~ 30 k^2 (eigenvalues), vectors and T the
+    # rest (flops / ms / 1e9 = TFLOP/s)
+    eig_total = nonempty * (9.0 * k ** 3 + 2.0 * k ** 3 + 4.0 * k * k)
+    e1 = nonempty * (4.0 / 3.0) * k ** 3
+    e2 = nonempty * 30.0 * k * k
+    e3 = eig_total - e1 - e2
     cand = {
-        "gram": ("mfma", gram_flop / (stages["ms_gram"] * 1e-3) / 1e12, peak_c[a.precision],
-                 "TFLOP/s", stages["ms_gram"]),
-        "solve": ("mfma", solve_flop / (stages["ms_solve"] * 1e-3) / 1e12, PEAK["fp64_TFs"],
-                  "TFLOP/s", stages["ms_solve"]),
-        "transform": ("hbm", xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
+        "gram": (gram_kernel, "mfma", gram_flop / (stages["ms_gram"] * 1e-3) / 1e12,
+                 peak_c[a.precision], "TFLOP/s", stages["ms_gram"]),
+        "eig_tridiag": ("k_trd", "mfma", e1 / max(stages["ms_eig_tridiag"], 1e-9) / 1e9,
+                        PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_tridiag"]),
+        "eig_values": ("k_trd_eigvals", "mfma", e2 / max(stages["ms_eig_values"], 1e-9) / 1e9,
+                       PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_values"]),
+        "eig_vectors": ("k_trd_vectors", "mfma", e3 / max(stages["ms_eig_vectors"], 1e-9) / 1e9,
+                        PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_vectors"]),
+        "transform": ("k_letkf_transform", "hbm", xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
                       PEAK["hbm_GBs"], "GB/s", stages["ms_transform"]),
     }
-    dom = max(cand, key=lambda kk: cand[kk][4])
-    b, ach, pk, unit, _ = cand[dom]
-    roof = {"kernel": {"gram": "k_letkf_gram_f64", "solve": "k_letkf_solve",
-                       "transform": "k_letkf_transform"}[dom],
-            "bound": b, "achieved": ach, "peak": pk, "unit": unit, "frac": ach / pk,
-            "traffic": None}
+    dom = max(cand, key=lambda kk: cand[kk][5])
+    kern, b, ach, pk, unit, _ = cand[dom]
+    roof = {"kernel": kern, "bound": b, "achieved": ach, "peak": pk, "unit": unit,
+            "frac": ach / pk, "traffic": None, "ms_per_launch": cand[dom][5]}
+    ratio_plain = (res["fp64_jacobi"]["ms_median"] / vres["ms_median"]
+                   if "fp64_jacobi" in res else None)
     out = {
         "metric": "LETKF analyses/sec at n=2.1e7,k=64: mixed-prec vs FP64 plain; RMS err vs FP64",
         "value": value, "unit": "analyses/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16x3": "bf16x3+f64"}[a.precision],
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16x3": "bf16x3+f64"}[a.precision],
         "data": "synthetic (seeded Gaussian random fields, uniform obs)",
         "config": {"workload": f"LETKF global lat-lon {a.nlon}x{a.nlat}x{nlev}, k={k}, "
                                f"p={p}, trilinear H, GC {a.radius:g} km, rho={a.rho}",
@@ -184,10 +266,13 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_ms": stages,
-        "eig_fallback_columns": d["n_fallback_columns"],
+        "speedup_vs_fp64_plain_jacobi": ratio_plain,
+        "speedup_vs_fp64_tridiagonal": res["fp64"]["ms_median"] / vres["ms_median"],
+        "paths": res,
         "local_obs": {"total": tot_loc, "mean": tot_loc / ncol, "max": d["max_local_obs"],
                       "empty_columns": d["n_empty_columns"]},
-        "parity": parity,
+        "parity": {"oracle": "standard LETKF (C, FP64)", "columns": len(cols),
+                   "max_rel_err_T": nonlocal_parity} if nonlocal_parity else None,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

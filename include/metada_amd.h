@@ -192,6 +192,13 @@ typedef struct {
   double ms_solve;      /* per-column eigensolve / Cholesky and T */
   double ms_transform;  /* X_a = xbar + X'T */
   double ms_total;
+  /* parts of ms_solve: tridiagonal reduction, eigenvalues, eigenvectors and
+   * T (zero when the tridiagonal path is not used), and the Jacobi /
+   * Cholesky kernel (fallback columns, or every column without that path) */
+  double ms_eig_tridiag;
+  double ms_eig_values;
+  double ms_eig_vectors;
+  double ms_solve_jacobi;
 } mda_letkf_diag;
 
 int mda_letkf_analyse(mda_ctx* ctx, const mda_letkf_params* prm, mda_ensemble* ens,

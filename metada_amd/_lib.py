@@ -72,7 +72,8 @@ class LetkfDiag(C.Structure):
                 ("max_local_obs", _i64), ("max_jacobi_sweeps", _i32),
                 ("n_fallback_columns", _i64), ("ms_obs", _d),
                 ("ms_bucket", _d), ("ms_gram", _d), ("ms_solve", _d), ("ms_transform", _d),
-                ("ms_total", _d)]
+                ("ms_total", _d), ("ms_eig_tridiag", _d), ("ms_eig_values", _d),
+                ("ms_eig_vectors", _d), ("ms_solve_jacobi", _d)]
 
 
 class Metrics(C.Structure):

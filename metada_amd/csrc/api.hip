@@ -407,7 +407,9 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     MDA_REQUIRE(std::isfinite(prm->inflation) && prm->inflation > 0.0, "inflation must be > 0");
     MDA_REQUIRE(std::isfinite(prm->radius) || prm->radius > 0.0, "radius must be > 0");
     MDA_REQUIRE(prm->radius > 0.0, "radius must be > 0");
-    MDA_REQUIRE(prm->precision == MDA_PREC_FP64, "this build implements the FP64 contractions");
+    MDA_REQUIRE(prm->precision == MDA_PREC_FP64 || prm->precision == MDA_PREC_FP32 ||
+                    prm->precision == MDA_PREC_BF16X3,
+                "precision must be FP64, FP32 or BF16X3");
     MDA_REQUIRE(geom->coord >= 0 && geom->coord <= 2, "unknown coordinate system");
     hipStream_t s = c->stream;
     MDA_HIP(hipEventRecord(c->ev[0], s));
@@ -490,6 +492,18 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     double* s_c = c->get<double>("s_c", p);
     launch_gather_obs(s, vals, p, k, yp, dd, o.var, o.a, o.b, o.c, prm->rweight, s_yp, s_d, s_r,
                       s_a, s_b, s_c);
+    float* s_y32 = nullptr;
+    double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
+    if (prm->precision != MDA_PREC_FP64) {
+      const bool geo = geom->coord == MDA_COORD_GEOGRAPHIC;
+      s_y32 = c->get<float>("s_y32", (size_t)p * k);
+      if (geo) {
+        s_ux = c->get<double>("s_ux", p);
+        s_uy = c->get<double>("s_uy", p);
+        s_uz = c->get<double>("s_uz", p);
+      }
+      launch_mixed_obs(s, p, k, s_yp, s_a, s_b, geo ? 1 : 0, s_y32, s_ux, s_uy, s_uz);
+    }
     MDA_HIP(hipEventRecord(c->ev[3], s));
 
     // 3. per-column Gram
@@ -502,6 +516,7 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     ga.grid = bg; ga.bstart = bstart;
     ga.yp = s_yp; ga.d = s_d; ga.rinv = s_r; ga.oa = s_a; ga.ob = s_b; ga.oc = s_c;
     ga.radius = prm->radius; ga.localization = prm->localization; ga.precision = prm->precision;
+    ga.y32 = s_y32; ga.ux = s_ux; ga.uy = s_uy; ga.uz = s_uz;
     ga.C = C; ga.b = bvec; ga.nloc = nloc;
     launch_letkf_gram(s, ga);
     MDA_HIP(hipEventRecord(c->ev[4], s));
@@ -520,6 +535,11 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     sa.lam = c->get<double>("eig_lam", (size_t)ncol * k);
     sa.flag = c->get<int>("eig_flag", ncol);
     sa.method = prm->eig_method;
+    sa.ev_stage[0] = c->ev[7]; sa.ev_stage[1] = c->ev[8]; sa.ev_stage[2] = c->ev[9];
+    // eigenvalue noise of the Gram relative to ||A||: FP32 products with FP32
+    // accumulation, or BF16x3 products (2^-16 relative) with FP32 accumulation
+    sa.null_rel = prm->precision == MDA_PREC_FP32 ? 0x1p-22
+                  : prm->precision == MDA_PREC_BF16X3 ? 0x1p-17 : 0.0;
     launch_zero(s, sweeps, sizeof(int) * ncol);
     launch_letkf_solve(s, sa);
     MDA_HIP(hipEventRecord(c->ev[5], s));
@@ -557,6 +577,15 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
       diag->ms_solve = elapsed(c->ev[4], c->ev[5]);
       diag->ms_transform = elapsed(c->ev[5], c->ev[6]);
       diag->ms_total = elapsed(c->ev[0], c->ev[6]);
+      if (eig_path) {
+        diag->ms_eig_tridiag = elapsed(c->ev[4], c->ev[7]);
+        diag->ms_eig_values = elapsed(c->ev[7], c->ev[8]);
+        diag->ms_eig_vectors = elapsed(c->ev[8], c->ev[9]);
+        diag->ms_solve_jacobi = elapsed(c->ev[9], c->ev[5]);
+      } else {
+        diag->ms_eig_tridiag = diag->ms_eig_values = diag->ms_eig_vectors = 0.0;
+        diag->ms_solve_jacobi = diag->ms_solve;
+      }
     }
   });
 }

@@ -223,8 +223,8 @@ template <int N>
 __global__ __launch_bounds__(64) void k_trd_vectors(
     int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
-    const double* __restrict__ rho_col, int prior, int combine, double* __restrict__ T,
-    int* __restrict__ flag) {
+    const double* __restrict__ rho_col, int prior, int combine, double null_rel,
+    double* __restrict__ T, int* __restrict__ flag) {
   constexpr int n = N;
   constexpr int NP = (N + 15) / 16 * 16;  // MFMA tile padding
   constexpr int ld = NP + 1;              // Z[i * ld + j]: component i of vector j
@@ -307,10 +307,12 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     //   wbar = s [ b / sigma - sum_j (1/sigma - 1/lambda_j)(z_j^T b) z_j ]
     // so eigenpairs with |lambda_j - sigma| <= 2e-12 sigma (the null space of
     // Y'^T R^-1 Y', where eigenvectors are ill-determined) need no vector:
-    // each would change W by at most 1e-12 relative.
+    // each would change W by at most 1e-12 relative.  With a reduced-precision
+    // Gram the window widens to the Gram's own rounding level, null_rel ||A||.
     const bool pri = prior != 0;
     const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
-    const bool skip = lane >= n || fabs(l - sigma) <= 2e-12 * sigma;
+    const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
+    const bool skip = lane >= n || fabs(l - sigma) <= null_tol;
     // weight of eigenpair j in W and wbar: 1 - sigma/lambda_j (0 at sigma)
     const double wj = skip ? 0.0 : 1.0 - sigma / l;
     sy[lane] = skip ? 1.0 : 0.0;  // skip flags, shared
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
         res = fmax(res, fabs(v));
       }
       if (!(res <= 1e-12 * tnorm)) why |= 2;
-      if (!(l >= sigma)) why |= 8;
+      if (!(l >= sigma - null_tol)) why |= 8;
     }
     if (lane < NP) {
 #pragma unroll
@@ -521,12 +523,15 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
                      prior, de, a.T);
   MDA_LAUNCH_CHECK();
+  if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
   hipLaunchKernelGGL(k_trd_eigvals<N>, dim3((unsigned)((a.ncol + 63) / 64)), dim3(64), 0, s,
                      a.ncol, a.nloc, de, lam, flag);
   MDA_LAUNCH_CHECK();
+  if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
   hipLaunchKernelGGL(k_trd_vectors<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.nloc, de, lam, a.b,
-                     a.rho, a.rho_col, prior, a.combine, a.T, flag);
+                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.T, flag);
   MDA_LAUNCH_CHECK();
+  if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }
 
 bool letkf_eig_supported(int k) {

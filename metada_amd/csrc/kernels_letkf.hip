@@ -253,6 +253,42 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Candidate window of column g: its location and the bucket rows / column
+// segments that can hold observations within the radius.
+struct col_window {
+  double ca, cb, cc;
+  int row0, row1;
+  seg_list sl;
+};
+
+__device__ inline col_window column_window(const letkf_gram_args& a, int64_t g) {
+  const bucket_grid& G = a.grid;
+  col_window w;
+  w.cc = 0.0;
+  if (a.col_a) { w.ca = a.col_a[g]; w.cb = a.col_b[g]; }
+  else { w.ca = (double)(g % a.nx); w.cb = (double)(g / a.nx); }
+  if (a.col_c) w.cc = a.col_c[g];
+  double r_lo, r_hi, c_lo, c_hi;
+  bool full = false;
+  if (a.coord == MDA_COORD_GEOGRAPHIC) {
+    const double dd = a.radius / kEarthRadiusKm;
+    const double ddeg = dd * 180.0 / kPi + 1e-9;
+    r_lo = w.ca - ddeg; r_hi = w.ca + ddeg;
+    const double cos_lat = cos(w.ca * kPi / 180.0);
+    full = r_lo <= -90.0 || r_hi >= 90.0 || sin(dd) >= cos_lat;
+    const double dl = full ? 180.0 : asin(fmin(1.0, sin(dd) / cos_lat)) * 180.0 / kPi + 1e-9;
+    const double lon = norm_lon(w.cb);
+    c_lo = lon - dl; c_hi = lon + dl;
+  } else {  // GRID / CARTESIAN: rows along y, bucket columns along x
+    r_lo = w.cb - a.radius - 1e-9; r_hi = w.cb + a.radius + 1e-9;
+    c_lo = w.ca - a.radius - 1e-9; c_hi = w.ca + a.radius + 1e-9;
+  }
+  w.row0 = clampi((int)floor((r_lo - G.a0) / G.da), 0, G.nrow - 1);
+  w.row1 = clampi((int)floor((r_hi - G.a0) / G.da), 0, G.nrow - 1);
+  w.sl = col_segments(G, c_lo, c_hi, full);
+  return w;
+}
+
 template <int KT>
 __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   constexpr int NT = KT * (KT + 1) / 2;
@@ -271,30 +307,10 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
 #pragma unroll
     for (int t = 0; t < KT; ++t) bp[t] = 0.0;
     int nl = 0;
-    // column location and rows of buckets in reach
-    double ca, cb, cc = 0.0;
-    if (a.col_a) { ca = a.col_a[g]; cb = a.col_b[g]; }
-    else { ca = (double)(g % a.nx); cb = (double)(g / a.nx); }
-    if (a.col_c) cc = a.col_c[g];
-    double r_lo, r_hi, c_lo, c_hi;
-    bool full = false;
-    double cos_lat = 0.0;
-    if (a.coord == MDA_COORD_GEOGRAPHIC) {
-      const double dd = a.radius / kEarthRadiusKm;
-      const double ddeg = dd * 180.0 / kPi + 1e-9;
-      r_lo = ca - ddeg; r_hi = ca + ddeg;
-      cos_lat = cos(ca * kPi / 180.0);
-      full = r_lo <= -90.0 || r_hi >= 90.0 || sin(dd) >= cos_lat;
-      double dl = full ? 180.0 : asin(fmin(1.0, sin(dd) / cos_lat)) * 180.0 / kPi + 1e-9;
-      double lon = norm_lon(cb);
-      c_lo = lon - dl; c_hi = lon + dl;
-    } else {  // GRID / CARTESIAN: rows along y, bucket columns along x
-      r_lo = cb - a.radius - 1e-9; r_hi = cb + a.radius + 1e-9;
-      c_lo = ca - a.radius - 1e-9; c_hi = ca + a.radius + 1e-9;
-    }
-    int row0 = clampi((int)floor((r_lo - G.a0) / G.da), 0, G.nrow - 1);
-    int row1 = clampi((int)floor((r_hi - G.a0) / G.da), 0, G.nrow - 1);
-    seg_list sl = col_segments(G, c_lo, c_hi, full);
+    const col_window cw = column_window(a, g);
+    const double ca = cw.ca, cb = cw.cb, cc = cw.cc;
+    const int row0 = cw.row0, row1 = cw.row1;
+    const seg_list sl = cw.sl;
     for (int row = row0; row <= row1; ++row) {
       for (int sg = 0; sg < sl.n; ++sg) {
         int64_t s0 = a.bstart[row * G.ncolb + sl.j0[sg]];
@@ -389,10 +405,350 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 2'. per-column Gram with FP32 / BF16x3 MFMA contractions (FP32 accumulate).
+// Same candidate search and weights as the FP64 kernel (FP64 geometry; the
+// GEOGRAPHIC distance from the chord of unit vectors, d = 2R asin(|u-v|/2),
+// equal to the haversine distance).  C = sum_o (sqrt(w_o) y'_o)(sqrt(w_o) y'_o)^T
+// so one scaled copy serves as both MFMA operands.  Compacted observations
+// are queued in LDS and consumed 32 at a time: lane m stages member m of the
+// 32 scaled rows (coalesced FP32 loads) into LDS as [member][obs];
+//   BF16x3: a = hi + lo (bf16, round to nearest even),
+//           C += hi hi^T + hi lo^T + lo hi^T on v_mfma_f32_16x16x32_bf16
+//           (A[l%16][8(l/16)+e], B[8(l/16)+e][l%16], D[4(l/16)+r][l%16]);
+//   FP32:   C += a a^T on v_mfma_f32_16x16x4f32 (A[l%16][l/16]).
+// b = sum_o w_o d_o y'_o accumulates in FP64 (lane m owns b_m).
+// Blocks are mapped XCD-contiguously: the dispatcher deals blocks to the 8
+// XCDs round-robin, so block i runs columns from slice (i % 8) and columns
+// that share observations share an L2.
+// ---------------------------------------------------------------------------
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ inline uint32_t bf16_rne_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+constexpr int GM_Q = 96;   // observation queue per wave (31 pending + 64 new)
+
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+constexpr int GM_LD = 40;  // bf16 row stride of the staging tile (32 + 8 pad)
+constexpr int GM_LDF = 36; // float row stride (32 + 4 pad)
+
+template <int KT, int PREC>
+__global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
+  constexpr int NT = KT * (KT + 1) / 2;
+  __shared__ int q_off[4][GM_Q];
+  __shared__ double q_wd[4][GM_Q];  // w_o d_o (for b)
+  __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
+  // staging: BF16x3 uses hi [64][GM_LD] + lo [64][GM_LD] bf16; FP32 [64][GM_LDF]
+  constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * 64 * GM_LD * 2 : 64 * GM_LDF * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char stg_raw[4][STG_BYTES];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int k = a.k;
+  const bucket_grid& G = a.grid;
+  const double cgc = a.radius / 2.0;
+  // chord threshold for GEOGRAPHIC: |u - v|^2 <= 4 sin^2(r / 2R)
+  const double half_ang = a.radius / (2.0 * kEarthRadiusKm);
+  const double chord2_r = half_ang >= kPi / 2.0 ? 5.0 : 4.0 * sin(half_ang) * sin(half_ang);
+  // XCD-contiguous virtual block index (gridDim.x is a multiple of 8)
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  unsigned char* stg = stg_raw[wv];
+  for (int64_t g = vb * 4 + wv; g < a.ncol; g += nb * 4) {
+    f4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+    double bacc = 0.0;
+    int nl = 0, pend = 0;
+    const col_window cw = column_window(a, g);
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (a.coord == MDA_COORD_GEOGRAPHIC) {
+      const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
+      vx = cos(la) * cos(lo); vy = cos(la) * sin(lo); vz = sin(la);
+    }
+    // consume the first 32 (or `cnt` < 32 at the end) queued observations
+    auto consume = [&](int cnt) {
+      // lane j < 32 holds queue entry j; entry j is broadcast to the wave by
+      // readlane (scalar registers), so the 32 row loads need no LDS reads.
+      // Entries j >= cnt read row 0 with zero scale and weight.
+      const int my_o = lane < cnt ? q_off[wv][lane] : 0;
+      const double my_wd = lane < cnt ? q_wd[wv][lane] : 0.0;
+      const float my_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
+      const float* yrow = a.y32 + (lane < k ? lane : 0);
+      float av[32];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int o = __builtin_amdgcn_readlane(my_o, j);
+        const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
+        const double wd = readlane_f64(my_wd, j);
+        const float y = lane < k ? yrow[(size_t)o * k] : 0.f;
+        bacc = fma(wd, (double)y, bacc);
+        av[j] = sw * y;
+      }
+      if (PREC == MDA_PREC_BF16X3) {
+        uint16_t* hi = (uint16_t*)stg;
+        uint16_t* lo = hi + 64 * GM_LD;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          uint32_t ph[4], pl[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float x0 = av[8 * q + 2 * e], x1 = av[8 * q + 2 * e + 1];
+            const uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
+            const uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
+            const uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+            ph[e] = h0 | (h1 << 16);
+            pl[e] = l0 | (l1 << 16);
+          }
+          *(uint4*)(hi + lane * GM_LD + 8 * q) = uint4{ph[0], ph[1], ph[2], ph[3]};
+          *(uint4*)(lo + lane * GM_LD + 8 * q) = uint4{pl[0], pl[1], pl[2], pl[3]};
+        }
+        wave_sync_lds();
+        bf8 fh[KT], fl[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          fh[t] = *(const bf8*)(hi + (16 * t + i16) * GM_LD + 8 * kq);
+          fl[t] = *(const bf8*)(lo + (16 * t + i16) * GM_LD + 8 * kq);
+        }
+        int tile = 0;
+#pragma unroll
+        for (int ta = 0; ta < KT; ++ta) {
+#pragma unroll
+          for (int tb = 0; tb <= ta; ++tb) {
+            acc[tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[ta], fh[tb], acc[tile], 0, 0, 0);
+            acc[tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[ta], fl[tb], acc[tile], 0, 0, 0);
+            acc[tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[ta], fh[tb], acc[tile], 0, 0, 0);
+            ++tile;
+          }
+        }
+      } else {
+        float* f = (float*)stg;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          *(float4*)(f + lane * GM_LDF + 4 * q) =
+              float4{av[4 * q], av[4 * q + 1], av[4 * q + 2], av[4 * q + 3]};
+        wave_sync_lds();
+#pragma unroll
+        for (int s4 = 0; s4 < 8; ++s4) {
+          float fr[KT];
+#pragma unroll
+          for (int t = 0; t < KT; ++t) fr[t] = f[(16 * t + i16) * GM_LDF + 4 * s4 + kq];
+          int tile = 0;
+#pragma unroll
+          for (int ta = 0; ta < KT; ++ta) {
+#pragma unroll
+            for (int tb = 0; tb <= ta; ++tb) {
+              acc[tile] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[ta], fr[tb], acc[tile], 0, 0, 0);
+              ++tile;
+            }
+          }
+        }
+      }
+      wave_sync_lds();
+    };
+    // segment bounds as scalars (no dynamically indexed local arrays)
+    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
+    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
+    for (int row = cw.row0; row <= cw.row1; ++row) {
+      for (int sg = 0; sg < cw.sl.n; ++sg) {
+        const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
+        const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
+        for (int64_t base = s0; base < s1; base += 64) {
+          const int64_t t = base + lane;
+          double w = 0.0;
+          bool in = false;
+          if (t < s1) {
+            double dist;
+            if (a.coord == MDA_COORD_GEOGRAPHIC) {
+              const double dx = vx - a.ux[t], dy = vy - a.uy[t], dz = vz - a.uz[t];
+              const double c2 = dx * dx + dy * dy + dz * dz;
+              in = c2 <= chord2_r;
+              dist = in ? 2.0 * kEarthRadiusKm * asin(fmin(1.0, 0.5 * sqrt(c2))) : 0.0;
+              in = in && dist <= a.radius;
+            } else {
+              const double dx = cw.ca - a.oa[t], dy = cw.cb - a.ob[t];
+              double d2 = dx * dx + dy * dy;
+              if (a.coord == MDA_COORD_CARTESIAN) {
+                const double dz = cw.cc - a.oc[t];
+                d2 += dz * dz;
+              }
+              dist = sqrt(d2);
+              in = dist <= a.radius;
+            }
+            // the GC polynomial rounds to tiny negative values just inside its
+            // support; those weights are zero here (sqrt(w) scales the rows)
+            if (in)
+              w = fmax((a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / cgc) : 1.0) *
+                           a.rinv[t], 0.0);
+          }
+          nl += __popcll(__ballot(in));
+          const uint64_t wm = __ballot(w != 0.0);
+          const int cnt = __popcll(wm);
+          if (cnt == 0) continue;
+          const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+          if (w != 0.0) {
+            const int slot = pend + __popcll(wm & below);
+            q_off[wv][slot] = (int)t;
+            q_wd[wv][slot] = w * a.d[t];
+            q_sw[wv][slot] = (float)sqrt(w);
+          }
+          pend += cnt;
+          wave_sync_lds();
+          while (pend >= 32) {
+            consume(32);
+            // move the remainder [32, pend) to the front of the queue
+            const int rest = pend - 32;
+            int mo = 0;
+            double mwd = 0.0;
+            float msw = 0.f;
+            if (lane < rest) { mo = q_off[wv][32 + lane]; mwd = q_wd[wv][32 + lane]; msw = q_sw[wv][32 + lane]; }
+            wave_sync_lds();
+            if (lane < rest) { q_off[wv][lane] = mo; q_wd[wv][lane] = mwd; q_sw[wv][lane] = msw; }
+            wave_sync_lds();
+            pend = rest;
+          }
+        }
+      }
+    }
+    if (pend > 0) consume(pend);
+    // Projection onto the complement of the ensemble-mean direction:
+    // Y'1 = 0 exactly, so C1 = 0 and 1^T b = 0 in exact arithmetic; the
+    // reduced-precision products break this at their rounding level and the
+    // eigenvalue (k-1)/rho of A along 1 would drift.  C <- PCP, b <- Pb with
+    // P = I - 11^T/k, from FP64 row sums of the accumulator tiles:
+    //   row i of tile (ta, tb <= ta): lanes of one 16-lane group, summed by
+    //   xor-shuffles over i16; the mirrored upper part of row i is column i
+    //   of the tiles below the diagonal, summed over kq and r.
+    double* rsum = (double*)stg;  // [64] FP64 row sums (staging is free now)
+    {
+      double rowp[KT][4], colp[KT];
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        colp[t] = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rowp[t][r] = 0.0;
+      }
+      int tile = 0;
+#pragma unroll
+      for (int ta = 0; ta < KT; ++ta) {
+#pragma unroll
+        for (int tb = 0; tb <= ta; ++tb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            rowp[ta][r] += (double)acc[tile][r];
+            if (tb != ta) colp[tb] += (double)acc[tile][r];
+          }
+          ++tile;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          for (int off = 1; off < 16; off <<= 1) rowp[t][r] += __shfl_xor(rowp[t][r], off);
+        colp[t] += __shfl_xor(colp[t], 16);
+        colp[t] += __shfl_xor(colp[t], 32);
+      }
+      if (i16 == 0) {
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) rsum[16 * t + 4 * kq + r] = rowp[t][r];
+      }
+      wave_sync_lds();
+      if (kq == 0) {
+#pragma unroll
+        for (int t = 0; t < KT; ++t) rsum[16 * t + i16] += colp[t];
+      }
+      wave_sync_lds();
+    }
+    double tsum = lane < k ? rsum[lane] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) tsum += __shfl_xor(tsum, off);
+    const double rk = 1.0 / (double)k;
+    const double tot = tsum * rk * rk;
+    double bsum = lane < k ? bacc : 0.0;
+    for (int off = 32; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
+    // b: lane m holds b_m
+    if (lane < k) a.b[(size_t)g * k + lane] = bacc - bsum * rk;
+    double* Cg = a.C + (size_t)g * k * k;
+    int tile = 0;
+#pragma unroll
+    for (int ta = 0; ta < KT; ++ta) {
+#pragma unroll
+      for (int tb = 0; tb <= ta; ++tb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * ta + 4 * kq + r, col = 16 * tb + i16;
+          if (row < k && col < k) {
+            const double v = (double)acc[tile][r] - (rsum[row] + rsum[col]) * rk + tot;
+            Cg[(size_t)row * k + col] = v;
+            if (ta != tb) Cg[(size_t)col * k + row] = v;
+          }
+        }
+        ++tile;
+      }
+    }
+    wave_sync_lds();  // rsum reads done before the next column's staging
+    if (lane == 0) a.nloc[g] = nl;
+  }
+}
+
+__global__ void k_mixed_obs(int64_t p, int k, const double* __restrict__ s_yp,
+                            const double* __restrict__ s_a, const double* __restrict__ s_b,
+                            int geographic, float* __restrict__ y32, double* __restrict__ ux,
+                            double* __restrict__ uy, double* __restrict__ uz) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < p * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    y32[e] = (float)s_yp[e];
+    if (geographic && e % k == 0) {
+      const int64_t t = e / k;
+      const double la = s_a[t] * kPi / 180.0, lo = s_b[t] * kPi / 180.0;
+      ux[t] = cos(la) * cos(lo); uy[t] = cos(la) * sin(lo); uz[t] = sin(la);
+    }
+  }
+}
+
+void launch_mixed_obs(hipStream_t s, int64_t p, int k, const double* s_yp, const double* s_a,
+                      const double* s_b, int geographic, float* y32, double* ux, double* uy,
+                      double* uz) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_mixed_obs, dim3(grid_for(p * k, 256, 65536)), dim3(256), 0, s, p, k,
+                     s_yp, s_a, s_b, geographic, y32, ux, uy, uz);
+  MDA_LAUNCH_CHECK();
+}
+
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   unsigned nb = (unsigned)((a.ncol + 3) / 4);
   if (nb > 262144) nb = 262144;
   int kt = (a.k + 15) / 16;
+  if (a.precision != MDA_PREC_FP64) {
+    MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "mixed Gram inputs missing");
+    nb = (nb + 7) / 8 * 8;  // XCD-contiguous mapping needs a multiple of 8
+    const bool bf = a.precision == MDA_PREC_BF16X3;
+#define MDA_GM(KT)                                                                         \
+  if (bf) hipLaunchKernelGGL((k_letkf_gram_mixed<KT, MDA_PREC_BF16X3>), dim3(nb), dim3(256), \
+                             0, s, a);                                                     \
+  else hipLaunchKernelGGL((k_letkf_gram_mixed<KT, MDA_PREC_FP32>), dim3(nb), dim3(256), 0, s, a);
+    switch (kt) {
+      case 1: MDA_GM(1); break;
+      case 2: MDA_GM(2); break;
+      case 3: MDA_GM(3); break;
+      case 4: MDA_GM(4); break;
+      default: throw mda_error(MDA_E_UNSUPPORTED, "LETKF Gram supports k <= 64 in this build");
+    }
+#undef MDA_GM
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   switch (kt) {
     case 1: hipLaunchKernelGGL(k_letkf_gram_f64<1>, dim3(nb), dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_letkf_gram_f64<2>, dim3(nb), dim3(256), 0, s, a); break;
@@ -560,7 +916,14 @@ __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X,
   __shared__ double xm[TR_L][TR_C];
   const int tid = threadIdx.x, nt = blockDim.x, ld = k + 1;
   const int64_t n = ncol * nlev;
-  const int64_t g0 = (int64_t)blockIdx.x * TR_C, l0 = (int64_t)blockIdx.y * TR_L;
+  // XCD-contiguous tile order (gridDim.x is a multiple of 8): the level tiles
+  // of one column tile run back to back on one XCD, so T_g is read from HBM
+  // once and from that XCD's L2 afterwards
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t nlt = (nlev + TR_L - 1) / TR_L;
+  if (vb >= ((ncol + TR_C - 1) / TR_C) * nlt) return;
+  const int64_t g0 = (vb / nlt) * TR_C, l0 = (vb % nlt) * TR_L;
   const int ncl = (int)(ncol - g0 < TR_C ? ncol - g0 : TR_C);
   const int nll = (int)(nlev - l0 < TR_L ? nlev - l0 : TR_L);
   for (int idx = tid; idx < k * TR_L * TR_C; idx += nt) {
@@ -624,8 +987,10 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
                             const double* T) {
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
   size_t shm = sizeof(double) * TR_L * TR_C * (k + 1);
-  dim3 grid((unsigned)((ncol + TR_C - 1) / TR_C), (unsigned)((nlev + TR_L - 1) / TR_L));
-  hipLaunchKernelGGL(k_letkf_transform, grid, dim3(256), shm, s, X, k, ncol, nlev, T);
+  const int64_t tiles = ((ncol + TR_C - 1) / TR_C) * ((nlev + TR_L - 1) / TR_L);
+  MDA_REQUIRE(tiles < (int64_t)1 << 31, "transform grid too large");
+  const unsigned nb = (unsigned)((tiles + 7) / 8 * 8);
+  hipLaunchKernelGGL(k_letkf_transform, dim3(nb), dim3(256), shm, s, X, k, ncol, nlev, T);
   MDA_LAUNCH_CHECK();
 }
 

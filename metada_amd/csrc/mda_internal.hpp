@@ -47,7 +47,7 @@ struct mda_ctx {
     size_t bytes = 0;
   };
   std::map<std::string, buf> ws;  // named device workspace, grown on demand
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
   // state of the last LETKF call (for mda_letkf_last_transforms)
   int last_k = 0;
   int64_t last_ncol = 0;
@@ -180,6 +180,11 @@ void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, int k,
                        const double* oa, const double* ob, const double* oc, int rweight,
                        double* s_yp, double* s_d, double* s_rinv, double* s_a, double* s_b,
                        double* s_c);
+// FP32 copy of the sorted Y' and unit vectors of sorted GEOGRAPHIC locations
+// (lat = s_a, lon = s_b in degrees), for the FP32 / BF16x3 Gram
+void launch_mixed_obs(hipStream_t s, int64_t p, int k, const double* s_yp, const double* s_a,
+                      const double* s_b, int geographic, float* y32, double* ux, double* uy,
+                      double* uz);
 struct letkf_gram_args {
   int k;
   int64_t ncol, nx;
@@ -198,6 +203,10 @@ struct letkf_gram_args {
   double radius;
   int localization;
   int precision;
+  const float* y32;    // sorted obs-major Y' in FP32 (FP32 / BF16x3 contractions)
+  const double* ux;    // sorted unit vectors of GEOGRAPHIC observation locations
+  const double* uy;
+  const double* uz;
   double* C;    // out [ncol][k][k]
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
@@ -220,6 +229,10 @@ struct letkf_solve_args {
   double* lam;   // [ncol][k]
   int* flag;     // [ncol] columns re-solved by the Jacobi fallback
   int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
+  hipEvent_t ev_stage[3];  // recorded after E1, E2, E3 when set (timing)
+  // Reduced-precision Gram: eigenvalues within null_rel * ||A|| of (k-1)/rho
+  // are treated as the null space of Y'^T R^-1 Y' (0 for FP64)
+  double null_rel;
 };
 void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
 // ensemble sizes with a compiled tridiagonal eigensolver (others use Jacobi)

@@ -308,3 +308,37 @@ def test_letkf_repeated_eigenvalue_falls_back_to_jacobi(mda, ctx, orc):
                        np.full(2, 0.25), f.inflation, f.localization_radius)
     Xo, _, _ = orc.letkf_standard(prob, X, hx)
     assert rel_err(ens.data, Xo) < TOL
+
+
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-6), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("k", [64, 40])
+def test_letkf_mixed_precision_contractions(mda, ctx, orc, prec, bound, k):
+    """FP32 / BF16x3 Y'^T R^-1 Y' on MFMA with the FP64 eigensolve: the
+    analysis ensemble stays within `bound` relative RMS of the oracle's FP64
+    analysis, and the mixed and FP64 GPU paths see the same local counts."""
+    case = latlon_case(k=k, p=6000, seed=70 + k, nmodes=96)
+    code = {"fp32": mda._lib.PREC_FP32, "bf16x3": mda._lib.PREC_BF16X3}[prec]
+    out = {}
+    for name, pc in (("fp64", mda._lib.PREC_FP64), (prec, code)):
+        ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+        obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                              coord=mda._lib.COORD_GEOGRAPHIC)
+        geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+        f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                       "localization_radius": 1800.0}),
+                      geometry=geom, ctx=ctx, precision=pc)
+        diag = f.Analyse()
+        out[name] = (ens.data.copy(), diag, f.local_counts())
+    prob = orc.Problem(k, case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert rel_err(out["fp64"][0], Xo) < TOL
+    np.testing.assert_array_equal(out["fp64"][2], out[prec][2])
+    assert out[prec][1]["total_local_obs"] == out["fp64"][1]["total_local_obs"]
+    Xm = out[prec][0]
+    rms = np.sqrt(np.mean((Xm - Xo) ** 2)) / np.sqrt(np.mean(Xo ** 2))
+    assert rms < bound, rms
+    # the increment itself is reproduced, not just the background
+    inc = np.sqrt(np.mean((Xo - case["X"]) ** 2))
+    assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 10 * bound * inc
