@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--skip-plain", action="store_true",
                     help="skip the FP64 Jacobi (plain) path")
+    ap.add_argument("--value-path-only", action="store_true",
+                    help="run only the --precision path (for profiler runs): no oracle, "
+                         "no FP64 reference, no error report")
     ap.add_argument("--nlon", type=int, default=1024)
     ap.add_argument("--nlat", type=int, default=512)
     ap.add_argument("--nlev", type=int, default=40)
@@ -78,19 +81,20 @@ def main():
     col_lat = W["geom"].col_a.cpu().numpy()
     col_lon = W["geom"].col_b.cpu().numpy()
     idx = (np.arange(nlev)[:, None] * ncol + cols[None, :]).ravel()       # [lev][sample]
-    Xs = W["X"][:, torch.from_numpy(idx).to(dev)].cpu().numpy()
-    hcols = W["H"].col.view(p, 8)
-    Yb = np.empty((k, p))
-    for j in range(k):  # H x_j member by member (bounded temporaries)
-        Yb[j] = (W["X"][j][hcols] * W["H"].weight.view(p, 8)).sum(-1).cpu().numpy()
-    log("oracle inputs staged on the host")
+    if not a.value_path_only:
+        Xs = W["X"][:, torch.from_numpy(idx).to(dev)].cpu().numpy()
+        hcols = W["H"].col.view(p, 8)
+        Yb = np.empty((k, p))
+        for j in range(k):  # H x_j member by member (bounded temporaries)
+            Yb[j] = (W["X"][j][hcols] * W["H"].weight.view(p, 8)).sum(-1).cpu().numpy()
+        log("oracle inputs staged on the host")
     yo = W["obs"].values.cpu().numpy()
     var = W["obs"].variances.cpu().numpy()
     rho32 = float(np.float32(a.rho))
     cpu = None
     T_oracle = None
     try:
-        if a.no_cpu_baseline:
+        if a.no_cpu_baseline or a.value_path_only:
             raise RuntimeError("skipped (--no-cpu-baseline)")
         from oracle import oracle as orc
         threads = a.threads or len(os.sched_getaffinity(0))
@@ -171,6 +175,14 @@ def main():
                 "ms_mean": float(np.mean(times)), "analyses_per_s": 1000.0 / float(np.median(times)),
                 "stages_ms": st, "fallback_columns": int(diags[-1]["n_fallback_columns"])}
 
+    if a.value_path_only:
+        code = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}[a.precision]
+        tv, dv = run(a.precision, make(code, L.EIG_AUTO), a.warmup, a.steps)
+        sv = summary(tv, dv)
+        print(json.dumps({"metric": "profiler run", "precision": a.precision,
+                          "ms_median": sv["ms_median"], "stages_ms": sv["stages_ms"]}), flush=True)
+        ctx.close()
+        return 0
     # FP64, tridiagonal eigensolver (fastest FP64 path): reference output
     f64 = make(L.PREC_FP64, L.EIG_AUTO)
     t64, d64 = run("fp64", f64, 1, max(1, min(a.steps, 5)), check_parity=True)

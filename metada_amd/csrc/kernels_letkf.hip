@@ -424,12 +424,8 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
 // ---------------------------------------------------------------------------
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-__device__ inline uint32_t bf16_rne_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
 
 constexpr int GM_Q = 96;   // observation queue per wave (31 pending + 64 new)
 
@@ -443,10 +439,11 @@ constexpr int GM_LD = 40;  // bf16 row stride of the staging tile (32 + 8 pad)
 constexpr int GM_LDF = 36; // float row stride (32 + 4 pad)
 
 template <int KT, int PREC>
-__global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_letkf_gram_mixed(letkf_gram_args a) {
   constexpr int NT = KT * (KT + 1) / 2;
   __shared__ int q_off[4][GM_Q];
-  __shared__ double q_wd[4][GM_Q];  // w_o d_o (for b)
+  __shared__ float q_wd[4][GM_Q];   // w_o d_o (for b)
   __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
   // staging: BF16x3 uses hi [64][GM_LD] + lo [64][GM_LD] bf16; FP32 [64][GM_LDF]
   constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * 64 * GM_LD * 2 : 64 * GM_LDF * 4;
@@ -455,7 +452,7 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
   const int i16 = lane & 15, kq = lane >> 4;
   const int k = a.k;
   const bucket_grid& G = a.grid;
-  const double cgc = a.radius / 2.0;
+  const float cgcf = (float)(a.radius / 2.0);
   // chord threshold for GEOGRAPHIC: |u - v|^2 <= 4 sin^2(r / 2R)
   const double half_ang = a.radius / (2.0 * kEarthRadiusKm);
   const double chord2_r = half_ang >= kPi / 2.0 ? 5.0 : 4.0 * sin(half_ang) * sin(half_ang);
@@ -479,39 +476,54 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
     auto consume = [&](int cnt) {
       // lane j < 32 holds queue entry j; entry j is broadcast to the wave by
       // readlane (scalar registers), so the 32 row loads need no LDS reads.
-      // Entries j >= cnt read row 0 with zero scale and weight.
+      // Entries j >= cnt read row 0 with zero scale and weight.  Rows are
+      // staged 8 at a time to keep few values live.  b takes FP32 partial
+      // sums over the 32 rows, added into the FP64 accumulator.
       const int my_o = lane < cnt ? q_off[wv][lane] : 0;
-      const double my_wd = lane < cnt ? q_wd[wv][lane] : 0.0;
+      const float my_wd = lane < cnt ? q_wd[wv][lane] : 0.f;
       const float my_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
       const float* yrow = a.y32 + (lane < k ? lane : 0);
-      float av[32];
+      float bp0 = 0.f, bp1 = 0.f;
 #pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        const int o = __builtin_amdgcn_readlane(my_o, j);
-        const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
-        const double wd = readlane_f64(my_wd, j);
-        const float y = lane < k ? yrow[(size_t)o * k] : 0.f;
-        bacc = fma(wd, (double)y, bacc);
-        av[j] = sw * y;
-      }
-      if (PREC == MDA_PREC_BF16X3) {
-        uint16_t* hi = (uint16_t*)stg;
-        uint16_t* lo = hi + 64 * GM_LD;
+      for (int c = 0; c < 4; ++c) {
+        float av[8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int e = 0; e < 8; ++e) {
+          const int j = 8 * c + e;
+          const int o = __builtin_amdgcn_readlane(my_o, j);
+          const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
+          const float wd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_wd), j));
+          const float y = lane < k ? yrow[(size_t)o * k] : 0.f;
+          if (e & 1) bp1 = fmaf(wd, y, bp1);
+          else bp0 = fmaf(wd, y, bp0);
+          av[e] = sw * y;
+        }
+        if (PREC == MDA_PREC_BF16X3) {
+          // a = hi + lo with hardware round-to-nearest-even conversions
           uint32_t ph[4], pl[4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float x0 = av[8 * q + 2 * e], x1 = av[8 * q + 2 * e + 1];
-            const uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
-            const uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
-            const uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
-            ph[e] = h0 | (h1 << 16);
-            pl[e] = l0 | (l1 << 16);
+          for (int q = 0; q < 4; ++q) {
+            const fl2 f = {av[2 * q], av[2 * q + 1]};
+            const bf2 h = __builtin_convertvector(f, bf2);
+            const fl2 r = f - __builtin_convertvector(h, fl2);
+            const bf2 lw = __builtin_convertvector(r, bf2);
+            ph[q] = *(const uint32_t*)&h;
+            pl[q] = *(const uint32_t*)&lw;
           }
-          *(uint4*)(hi + lane * GM_LD + 8 * q) = uint4{ph[0], ph[1], ph[2], ph[3]};
-          *(uint4*)(lo + lane * GM_LD + 8 * q) = uint4{pl[0], pl[1], pl[2], pl[3]};
+          uint16_t* hi = (uint16_t*)stg;
+          uint16_t* lo = hi + 64 * GM_LD;
+          *(uint4*)(hi + lane * GM_LD + 8 * c) = uint4{ph[0], ph[1], ph[2], ph[3]};
+          *(uint4*)(lo + lane * GM_LD + 8 * c) = uint4{pl[0], pl[1], pl[2], pl[3]};
+        } else {
+          float* f = (float*)stg;
+          *(float4*)(f + lane * GM_LDF + 8 * c) = float4{av[0], av[1], av[2], av[3]};
+          *(float4*)(f + lane * GM_LDF + 8 * c + 4) = float4{av[4], av[5], av[6], av[7]};
         }
+      }
+      bacc += (double)bp0 + (double)bp1;
+      if (PREC == MDA_PREC_BF16X3) {
+        const uint16_t* hi = (const uint16_t*)stg;
+        const uint16_t* lo = hi + 64 * GM_LD;
         wave_sync_lds();
         bf8 fh[KT], fl[KT];
 #pragma unroll
@@ -531,11 +543,7 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
           }
         }
       } else {
-        float* f = (float*)stg;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          *(float4*)(f + lane * GM_LDF + 4 * q) =
-              float4{av[4 * q], av[4 * q + 1], av[4 * q + 2], av[4 * q + 3]};
+        const float* f = (const float*)stg;
         wave_sync_lds();
 #pragma unroll
         for (int s4 = 0; s4 < 8; ++s4) {
@@ -564,16 +572,18 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
         const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
         for (int64_t base = s0; base < s1; base += 64) {
           const int64_t t = base + lane;
-          double w = 0.0;
+          float w = 0.f;
           bool in = false;
           if (t < s1) {
-            double dist;
+            // membership in FP64 (the chord test c2 <= 4 sin^2(r/2R) is the
+            // haversine test d <= r); the weight in FP32, the precision of
+            // the contraction it feeds
+            float dist;
             if (a.coord == MDA_COORD_GEOGRAPHIC) {
               const double dx = vx - a.ux[t], dy = vy - a.uy[t], dz = vz - a.uz[t];
               const double c2 = dx * dx + dy * dy + dz * dz;
               in = c2 <= chord2_r;
-              dist = in ? 2.0 * kEarthRadiusKm * asin(fmin(1.0, 0.5 * sqrt(c2))) : 0.0;
-              in = in && dist <= a.radius;
+              dist = 2.f * (float)kEarthRadiusKm * asinf(fminf(1.f, 0.5f * sqrtf((float)c2)));
             } else {
               const double dx = cw.ca - a.oa[t], dy = cw.cb - a.ob[t];
               double d2 = dx * dx + dy * dy;
@@ -581,25 +591,27 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
                 const double dz = cw.cc - a.oc[t];
                 d2 += dz * dz;
               }
-              dist = sqrt(d2);
-              in = dist <= a.radius;
+              in = sqrt(d2) <= a.radius;
+              dist = (float)sqrt(d2);
             }
             // the GC polynomial rounds to tiny negative values just inside its
             // support; those weights are zero here (sqrt(w) scales the rows)
             if (in)
-              w = fmax((a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / cgc) : 1.0) *
-                           a.rinv[t], 0.0);
+              w = fmaxf((a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn_f32(dist / cgcf)
+                                                                : 1.f) *
+                            (float)a.rinv[t],
+                        0.f);
           }
           nl += __popcll(__ballot(in));
-          const uint64_t wm = __ballot(w != 0.0);
+          const uint64_t wm = __ballot(w != 0.f);
           const int cnt = __popcll(wm);
           if (cnt == 0) continue;
           const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-          if (w != 0.0) {
+          if (w != 0.f) {
             const int slot = pend + __popcll(wm & below);
             q_off[wv][slot] = (int)t;
-            q_wd[wv][slot] = w * a.d[t];
-            q_sw[wv][slot] = (float)sqrt(w);
+            q_wd[wv][slot] = w * (float)a.d[t];
+            q_sw[wv][slot] = sqrtf(w);
           }
           pend += cnt;
           wave_sync_lds();
@@ -608,8 +620,7 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
             // move the remainder [32, pend) to the front of the queue
             const int rest = pend - 32;
             int mo = 0;
-            double mwd = 0.0;
-            float msw = 0.f;
+            float mwd = 0.f, msw = 0.f;
             if (lane < rest) { mo = q_off[wv][32 + lane]; mwd = q_wd[wv][32 + lane]; msw = q_sw[wv][32 + lane]; }
             wave_sync_lds();
             if (lane < rest) { q_off[wv][lane] = mo; q_wd[wv][lane] = mwd; q_sw[wv][lane] = msw; }
@@ -623,19 +634,20 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
     // Projection onto the complement of the ensemble-mean direction:
     // Y'1 = 0 exactly, so C1 = 0 and 1^T b = 0 in exact arithmetic; the
     // reduced-precision products break this at their rounding level and the
-    // eigenvalue (k-1)/rho of A along 1 would drift.  C <- PCP, b <- Pb with
-    // P = I - 11^T/k, from FP64 row sums of the accumulator tiles:
-    //   row i of tile (ta, tb <= ta): lanes of one 16-lane group, summed by
+    // eigenvalue (k-1)/rho of A along 1 would drift below the null window.
+    // C <- PCP, b <- Pb with P = I - 11^T/k.  Row sums in FP32 (their error,
+    // ~1e-7 ||C||, is far inside the null window):
+    //   row i of tile (ta, tb <= ta): the 16 lanes of one kq group, summed by
     //   xor-shuffles over i16; the mirrored upper part of row i is column i
     //   of the tiles below the diagonal, summed over kq and r.
-    double* rsum = (double*)stg;  // [64] FP64 row sums (staging is free now)
+    double* rsum = (double*)stg;  // [64] row sums (staging is free now)
     {
-      double rowp[KT][4], colp[KT];
+      float rowp[KT][4], colp[KT];
 #pragma unroll
       for (int t = 0; t < KT; ++t) {
-        colp[t] = 0.0;
+        colp[t] = 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) rowp[t][r] = 0.0;
+        for (int r = 0; r < 4; ++r) rowp[t][r] = 0.f;
       }
       int tile = 0;
 #pragma unroll
@@ -644,8 +656,8 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
         for (int tb = 0; tb <= ta; ++tb) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            rowp[ta][r] += (double)acc[tile][r];
-            if (tb != ta) colp[tb] += (double)acc[tile][r];
+            rowp[ta][r] += acc[tile][r];
+            if (tb != ta) colp[tb] += acc[tile][r];
           }
           ++tile;
         }
@@ -662,12 +674,12 @@ __global__ __launch_bounds__(256) void k_letkf_gram_mixed(letkf_gram_args a) {
 #pragma unroll
         for (int t = 0; t < KT; ++t)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) rsum[16 * t + 4 * kq + r] = rowp[t][r];
+          for (int r = 0; r < 4; ++r) rsum[16 * t + 4 * kq + r] = (double)rowp[t][r];
       }
       wave_sync_lds();
       if (kq == 0) {
 #pragma unroll
-        for (int t = 0; t < KT; ++t) rsum[16 * t + i16] += colp[t];
+        for (int t = 0; t < KT; ++t) rsum[16 * t + i16] += (double)colp[t];
       }
       wave_sync_lds();
     }

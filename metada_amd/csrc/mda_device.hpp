@@ -31,6 +31,15 @@ __device__ inline double gaspari_cohn(double z) {
          2.0 / (3.0 * z);
 }
 
+// FP32 form of gaspari_cohn, for weights that feed FP32 / BF16 contractions.
+__device__ inline float gaspari_cohn_f32(float z) {
+  z = fabsf(z);
+  if (z >= 2.f) return 0.f;
+  if (z <= 1.f) return (((-0.25f * z + 0.5f) * z + 0.625f) * z - 5.f / 3.f) * z * z + 1.f;
+  return ((((z / 12.f - 0.5f) * z + 0.625f) * z + 5.f / 3.f) * z - 5.f) * z + 4.f -
+         2.f / (3.f * z);
+}
+
 // ---------------------------------------------------------------------------
 // workgroup reductions
 // ---------------------------------------------------------------------------
