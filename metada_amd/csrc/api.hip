@@ -533,6 +533,10 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     sa.combine = prm->combine; sa.T = T; sa.sweeps = sweeps; sa.status = status;
     sa.de = c->get<double>("eig_de", (size_t)ncol * 2 * k);
     sa.lam = c->get<double>("eig_lam", (size_t)ncol * k);
+    sa.U = letkf_eig_supported(k) && prm->eig_method == MDA_EIG_AUTO &&
+                   prm->sqrt_form == MDA_SQRT_SYMMETRIC
+               ? c->get<double>("eig_U", (size_t)ncol * k * k)
+               : nullptr;
     sa.flag = c->get<int>("eig_flag", ncol);
     sa.method = prm->eig_method;
     sa.ev_stage[0] = c->ev[7]; sa.ev_stage[1] = c->ev[8]; sa.ev_stage[2] = c->ev[9];

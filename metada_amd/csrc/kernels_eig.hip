@@ -2,10 +2,8 @@
 // square root of LETKF.hpp:223-224 in its standard form, W = sqrt(k-1) A^-1/2).
 //
 //   E1  k_trd: Householder reduction A = Q T Q^T to tridiagonal form, one
-//       wavefront per column with A in LDS.  d, e go to `de`; the Householder
-//       vectors u_j and tau_j are parked in the column's slot of the T buffer
-//       (that slot is only written with the final transform by the same
-//       column's E3 wavefront, after it has consumed them).
+//       wavefront per column with rows of A in registers.  d, e go to `de`;
+//       the Householder vectors u_j and tau_j go to the reflector buffer U.
 //   E2  k_trd_eigvals: eigenvalues of T by implicit QL with Wilkinson shifts,
 //       one LANE per column so the serial scalar recurrence of 64 columns runs
 //       side by side.
@@ -53,10 +51,11 @@ __device__ inline double wave_sum(double v) {
 // U[j][0] (never a reflector position).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(64) void k_trd(int64_t ncol, const double* __restrict__ C,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_trd(int64_t ncol, const double* __restrict__ C,
                                             const int* __restrict__ nloc, double rho,
                                             const double* __restrict__ rho_col, int prior,
-                                            double* __restrict__ de, double* __restrict__ Tslot) {
+                                            double* __restrict__ de, double* __restrict__ Uout) {
   constexpr int n = N;
   __shared__ double row[EIG_N], ub[EIG_N], wb[EIG_N];
   const int lane = threadIdx.x;
@@ -72,39 +71,57 @@ __global__ __launch_bounds__(64) void k_trd(int64_t ncol, const double* __restri
       double v = Cg[(size_t)rl * n + c];
       ar[c] = lane < n ? v + (lane == c ? sigma : 0.0) : 0.0;
     }
-    double* U = Tslot + (size_t)g * n * n;
+    double* U = Uout + (size_t)g * n * n;
     double* dg = de + (size_t)g * 2 * n;
-    for (int j = 0; j < n - 2; ++j) {
-      if (lane == j) {
+    // Steps j in chunks of 16: within chunk ch every quantity indexed by a
+    // column c <= j is zero or unused, so the static column ranges start at
+    // 16 ch (fewer row writes, dot-product and update terms).
+    constexpr int NCH = (N - 2 + 15) / 16;
 #pragma unroll
-        for (int c = 0; c < N; ++c) row[c] = ar[c];
+    for (int ch = 0; ch < NCH; ++ch) {
+      const int c0 = 16 * ch;
+      const int jend = (16 * ch + 16 < n - 2) ? 16 * ch + 16 : n - 2;
+#pragma unroll 1
+      for (int j = c0; j < jend; ++j) {
+        if (lane == j) {
+#pragma unroll
+          for (int c = c0; c < N; ++c) row[c] = ar[c];
+        }
+        wave_lds_sync();
+        const double xi = (lane > j && lane < n) ? row[lane] : 0.0;  // A[lane][j] = A[j][lane]
+        const double nrm2 = wave_sum(xi * xi);
+        const double x0 = row[j + 1];
+        double alpha = 0.0, tau = 0.0, ui = 0.0;
+        if (nrm2 > 0.0) {
+          alpha = (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2);
+          ui = xi - (lane == j + 1 ? alpha : 0.0);
+          tau = 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha);
+        }
+        if (lane == 0) { dg[j] = row[j]; dg[n + j] = alpha; }
+        if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
+        if (tau == 0.0) { wave_lds_sync(); continue; }  // x = 0: H = I
+        ub[lane] = ui;
+        wave_lds_sync();
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+        for (int c = c0; c < N; c += 4) {
+          p0 += ar[c] * ub[c];
+          if (c + 1 < N) p1 += ar[c + 1 < N ? c + 1 : c] * ub[c + 1 < N ? c + 1 : c];
+          if (c + 2 < N) p2 += ar[c + 2 < N ? c + 2 : c] * ub[c + 2 < N ? c + 2 : c];
+          if (c + 3 < N) p3 += ar[c + 3 < N ? c + 3 : c] * ub[c + 3 < N ? c + 3 : c];
+        }
+        const double p = tau * ((p0 + p1) + (p2 + p3));
+        const double K = 0.5 * tau * wave_sum(p * ui);
+        const double wi = (lane > j && lane < n) ? p - K * ui : 0.0;
+        wb[lane] = wi;
+        wave_lds_sync();
+#pragma unroll
+        for (int c = c0; c < N; ++c) {
+          if ((c & 7) == 0) lds_reload_point();
+          ar[c] -= ui * wb[c] + wi * ub[c];
+        }
+        wave_lds_sync();
       }
-      wave_lds_sync();
-      const double xi = (lane > j && lane < n) ? row[lane] : 0.0;  // A[lane][j] = A[j][lane]
-      const double nrm2 = wave_sum(xi * xi);
-      const double x0 = row[j + 1];
-      double alpha = 0.0, tau = 0.0, ui = 0.0;
-      if (nrm2 > 0.0) {
-        alpha = (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2);
-        ui = xi - (lane == j + 1 ? alpha : 0.0);
-        tau = 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha);
-      }
-      if (lane == 0) { dg[j] = row[j]; dg[n + j] = alpha; }
-      if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
-      if (tau == 0.0) { wave_lds_sync(); continue; }  // x = 0: H = I
-      ub[lane] = ui;
-      wave_lds_sync();
-      double p = 0.0;
-#pragma unroll
-      for (int c = 0; c < N; ++c) p += ar[c] * ub[c];
-      p *= tau;
-      const double K = 0.5 * tau * wave_sum(p * ui);
-      const double wi = (lane > j && lane < n) ? p - K * ui : 0.0;
-      wb[lane] = wi;
-      wave_lds_sync();
-#pragma unroll
-      for (int c = 0; c < N; ++c) ar[c] -= ui * wb[c] + wi * ub[c];
-      wave_lds_sync();
     }
     // last 2x2 block: rows n-2 and n-1 through the LDS row
     if (lane == n - 2) {
@@ -224,7 +241,7 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
     const double* __restrict__ rho_col, int prior, int combine, double null_rel,
-    double* __restrict__ T, int* __restrict__ flag) {
+    const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
   constexpr int n = N;
   constexpr int NP = (N + 15) / 16 * 16;  // MFMA tile padding
   constexpr int ld = NP + 1;              // Z[i * ld + j]: component i of vector j
@@ -391,18 +408,12 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     }
     // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
     // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
-    // r = 0) are staged in LDS with coalesced loads and read as broadcasts.
-    // Reflectors are taken in chunks of 16 so the static inner range starts
-    // at the chunk's first non-zero position.
+    // r = 0) are the same for every lane: they are read through the scalar
+    // cache into SGPRs and used directly as FMA operands.  Reflectors are
+    // taken in chunks of 16 so the static inner range starts at the chunk's
+    // first non-zero position.
     {
-      double* Us = Z;
-      constexpr int NN = N * N;
-#pragma unroll
-      for (int t = 0; t < (NN + 63) / 64; ++t) {
-        const int idx = t * 64 + lane;
-        if (idx < NN) Us[idx] = Tg[idx];
-      }
-      wave_lds_sync();
+      const double* Ucol = Ug + (size_t)g * n * n;
       constexpr int NC = (N - 2 + 15) / 16;
 #pragma unroll
       for (int c = NC - 1; c >= 0; --c) {
@@ -410,12 +421,11 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
         const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
 #pragma unroll 1
         for (int j = jhi; j >= jlo; --j) {
-          const double* u = Us + j * N;
+          const double* u = Ucol + j * N;
           const double tau = u[0];
           double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
           for (int i = jlo + 1; i < N; ++i) {
-            if ((i & 15) == 0) lds_reload_point();
             const double t = u[i] * z[i];
             switch (i & 3) {
               case 0: p0 += t; break;
@@ -426,13 +436,9 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
           }
           const double dot = tau * ((p0 + p1) + (p2 + p3));
 #pragma unroll
-          for (int i = jlo + 1; i < N; ++i) {
-            if ((i & 15) == 0 || i == jlo + 1) lds_reload_point();
-            z[i] -= dot * u[i];
-          }
+          for (int i = jlo + 1; i < N; ++i) z[i] -= dot * u[i];
         }
       }
-      wave_lds_sync();
     }
     // ---- U_s = Z diag(sqrt(g)) and y' for wbar ----
     const double sc = pri ? 1.0 : r;
@@ -521,7 +527,7 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   launch_zero(s, flag, sizeof(int) * a.ncol);
   const unsigned nb = (unsigned)(a.ncol < 131072 ? a.ncol : 131072);
   hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
-                     prior, de, a.T);
+                     prior, de, a.U);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
   hipLaunchKernelGGL(k_trd_eigvals<N>, dim3((unsigned)((a.ncol + 63) / 64)), dim3(64), 0, s,
@@ -529,7 +535,7 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
   hipLaunchKernelGGL(k_trd_vectors<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.nloc, de, lam, a.b,
-                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.T, flag);
+                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.T, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }

@@ -995,9 +995,96 @@ __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X,
   }
 }
 
+// 4'. X_a = xbar + X' T_g with T_g held in registers: thread (c, j) keeps
+// column j of T for column g0 + c (K doubles) and sweeps the levels, so T
+// is read from memory exactly once per analysis.  A block covers XT_C = 4
+// consecutive columns and all K members; levels are staged through LDS XT_L
+// at a time as [level][column][member].  One pass over a staged row gives
+// both the member sum s and the product p = sum_m x_m T[m][j]; with
+// xbar = s / k and sum_m (x_m - xbar) T[m][j] = p - xbar colsum_j,
+//   X_a[j] = xbar (1 - colsum_j) + p.
+// Lane layout: c = lane % 4, j = 16 * wave + lane / 4, so a store
+// instruction writes 16 members x 32 B.
+constexpr int XT_C = 4, XT_L = 20;
+
+template <int K>
+__global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict__ X,
+                                                             int64_t ncol, int64_t nlev,
+                                                             const double* __restrict__ T) {
+  constexpr int KP = K + 2;  // row pad: 16-byte aligned, banks shift by 4 per row
+  extern __shared__ double xs[];  // [XT_L][XT_C][KP]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c = lane % XT_C, j = 16 * wv + lane / XT_C;
+  const int64_t n = ncol * nlev;
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t g0 = vb * XT_C;
+  if (g0 >= ncol) return;
+  const int ncl = (int)(ncol - g0 < XT_C ? ncol - g0 : XT_C);
+  const bool own = j < K && c < ncl;
+  double t[K];
+  double colsum = 0.0;
+  {
+    const double* Tg = T + (size_t)(g0 + (c < ncl ? c : 0)) * K * K + (j < K ? j : 0);
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      t[m] = own ? Tg[(size_t)m * K] : 0.0;
+      colsum += t[m];
+    }
+  }
+  const double rk = 1.0 / (double)K;
+  const double cmean = 1.0 - colsum;
+  for (int64_t l0 = 0; l0 < nlev; l0 += XT_L) {
+    const int nll = (int)(nlev - l0 < XT_L ? nlev - l0 : XT_L);
+    __syncthreads();  // previous chunk's readers are done with xs
+    for (int idx = tid; idx < K * XT_L * XT_C; idx += 256) {
+      const int cc = idx % XT_C, l = (idx / XT_C) % XT_L, m = idx / (XT_C * XT_L);
+      if (cc < ncl && l < nll)
+        xs[(l * XT_C + cc) * KP + m] = X[(size_t)m * n + (l0 + l) * ncol + g0 + cc];
+    }
+    __syncthreads();
+    if (own) {
+      for (int l = 0; l < nll; ++l) {
+        const double* row = xs + (l * XT_C + c) * KP;
+        double a0 = 0.0, a1 = 0.0, s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < K; m += 2) {
+          a0 += row[m] * t[m];
+          s0 += row[m];
+          if (m + 1 < K) {
+            a1 += row[m + 1 < K ? m + 1 : m] * t[m + 1 < K ? m + 1 : m];
+            s1 += row[m + 1 < K ? m + 1 : m];
+          }
+        }
+        X[(size_t)j * n + (l0 + l) * ncol + g0 + c] = (s0 + s1) * rk * cmean + (a0 + a1);
+      }
+    }
+  }
+}
+
+bool letkf_transform_reg_supported(int k) {
+  switch (k) {
+    case 8: case 16: case 20: case 32: case 40: case 48: case 64: return true;
+    default: return false;
+  }
+}
+
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
                             const double* T) {
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
+  if (letkf_transform_reg_supported(k)) {
+    MDA_REQUIRE((ncol + XT_C - 1) / XT_C < (int64_t)1 << 31, "transform grid too large");
+    const unsigned nbr = (unsigned)(((ncol + XT_C - 1) / XT_C + 7) / 8 * 8);
+    const size_t shr = sizeof(double) * XT_L * XT_C * (k + 2);
+#define MDA_XT(KK)                                                                          \
+  case KK:                                                                                 \
+    hipLaunchKernelGGL(k_letkf_transform_reg<KK>, dim3(nbr), dim3(256), shr, s, X, ncol, nlev, T); \
+    break;
+    switch (k) { MDA_XT(8) MDA_XT(16) MDA_XT(20) MDA_XT(32) MDA_XT(40) MDA_XT(48) MDA_XT(64) }
+#undef MDA_XT
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   size_t shm = sizeof(double) * TR_L * TR_C * (k + 1);
   const int64_t tiles = ((ncol + TR_C - 1) / TR_C) * ((nlev + TR_L - 1) / TR_L);
   MDA_REQUIRE(tiles < (int64_t)1 << 31, "transform grid too large");

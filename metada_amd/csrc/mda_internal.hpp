@@ -227,6 +227,7 @@ struct letkf_solve_args {
   // tridiagonal eigensolver workspace (symmetric square root, k <= 64)
   double* de;    // [ncol][2k]
   double* lam;   // [ncol][k]
+  double* U;     // [ncol][k][k] Householder reflectors
   int* flag;     // [ncol] columns re-solved by the Jacobi fallback
   int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
   hipEvent_t ev_stage[3];  // recorded after E1, E2, E3 when set (timing)
