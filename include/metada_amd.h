@@ -221,6 +221,48 @@ int mda_letkf_last_local_counts(mda_ctx* ctx, int32_t* counts);
  * close eigenpair, 8 eigenvalue below (k-1)/rho). */
 int mda_letkf_last_column_flags(mda_ctx* ctx, int32_t* flags);
 
+/* ---- Learned inflation (no counterpart in the reference) -------------------------
+ * Cycled twin experiments and a per-column MLP that predicts the LETKF
+ * inflation rho_g (fed to mda_letkf_params.rho_column).  The forecast model
+ * is Lorenz-96 (tests/python/lorenz95.py:3-37).
+ */
+
+/* Advances `nstates` Lorenz-96 states of size N (row-major [nstates][N]) in
+ * place by `nsteps` steps of dt: scheme 0 = classical RK4, 1 = the forward
+ * Euler step of lorenz95.py:28 with its clipping to +-1e10.  4 <= N <= 6144. */
+int mda_l96_forecast(mda_ctx* ctx, double* X, int64_t nstates, int64_t N, double forcing,
+                     double dt, int32_t nsteps, int32_t scheme, int32_t on_device);
+
+/* Eight local innovation and spread statistics of every column, FP32
+ * [nx*ny][8], from the background ensemble and observations (same local
+ * observation sets and weights as the LETKF with these params):
+ * log(Sd2/Sw), log(Ssp/Sw), log(Sd2/(Ssp+Sw)), Sd/sqrt(Sw Sd2), log(Sw),
+ * log(1+n_local), log(column background variance), rho_prev - 1, where
+ * Sw = sum w, Sd2 = sum w d^2, Sd = sum w d, Ssp = sum w |y'|^2/(k-1) over the
+ * column's local observations with w = localisation / variance.
+ * rho_prev (device or host per prm->rho_on_device, may be NULL) is the
+ * previous inflation of each column. */
+int mda_letkf_inflation_features(mda_ctx* ctx, const mda_letkf_params* prm,
+                                 const mda_ensemble* ens, const mda_geometry* geom,
+                                 const mda_obs* obs, const mda_hop* hop, const double* rho_prev,
+                                 float* features, int32_t features_on_device);
+
+/* MLP 8 -> 64 -> 64 -> 1 (tanh, rho = 1 + softplus(out)), FP32.  Parameters:
+ * MDA_MLP_NPARAM floats laid out W1[64][8], b1[64], W2[64][64], b2[64],
+ * W3[64], b3.  All arrays on the device when on_device, else host. */
+enum { MDA_MLP_NIN = 8, MDA_MLP_HIDDEN = 64, MDA_MLP_NPARAM = 4801 };
+int mda_mlp_forward(mda_ctx* ctx, const float* params, const float* features, int64_t m,
+                    float* rho, int32_t on_device);
+/* Mean squared error of rho against `target` over m samples and its gradient
+ * (grad: MDA_MLP_NPARAM floats); the loss is returned on the host. */
+int mda_mlp_loss_grad(mda_ctx* ctx, const float* params, const float* features,
+                      const float* target, int64_t m, double* loss, float* grad,
+                      int32_t on_device);
+/* One Adam step (step counts from 1) on params with moments m1, m2. */
+int mda_mlp_adam_step(mda_ctx* ctx, float* params, float* m1, float* m2, const float* grad,
+                      int64_t step, float lr, float beta1, float beta2, float eps,
+                      int32_t on_device);
+
 /* ---- Metrics (Metrics.hpp:74-294) ------------------------------------------------ */
 typedef struct {
   double rmse, bias, correlation, crps, avg_spread;

@@ -109,7 +109,19 @@ SIGNATURES = {
     "mda_letkf_last_column_flags": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mda_ensemble_metrics": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), C.c_void_p, _i32,
                                        C.POINTER(Metrics)]),
+    "mda_l96_forecast": (C.c_int, [C.c_void_p, C.c_void_p, _i64, _i64, _d, _d, _i32, _i32,
+                                   _i32]),
+    "mda_letkf_inflation_features": (C.c_int, [C.c_void_p, C.POINTER(LetkfParams),
+                                               C.POINTER(Ensemble), C.POINTER(Geometry),
+                                               C.POINTER(Obs), C.POINTER(Hop), C.c_void_p,
+                                               C.c_void_p, _i32]),
+    "mda_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_void_p, _i32]),
+    "mda_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i64,
+                                    C.POINTER(_d), C.c_void_p, _i32]),
+    "mda_mlp_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i32]),
 }
+MLP_NIN, MLP_HIDDEN, MLP_NPARAM = 8, 64, 4801
 
 _lib = None
 

@@ -399,6 +399,118 @@ void mda_letkf_params_refcompat(mda_letkf_params* p, double inflation, double ra
   p->precision = MDA_PREC_FP64;
 }
 
+// Geometry staging, observation-space statistics (HX, Y', d), the bucket
+// index over the localisation coordinates and the gather of the observation
+// data into bucket order: everything before the per-column work of the
+// LETKF (LETKF.hpp:152-211).  Records ev[1..3].
+struct local_obs {
+  const double *ca = nullptr, *cb = nullptr, *cc = nullptr, *rho_col = nullptr;
+  bucket_grid bg{};
+  const int* bstart = nullptr;
+  double *s_yp = nullptr, *s_d = nullptr, *s_r = nullptr, *s_a = nullptr, *s_b = nullptr,
+         *s_c = nullptr;
+  float* s_y32 = nullptr;
+  double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
+};
+
+static local_obs prepare_local(mda_ctx* c, const mda_letkf_params* prm, const dev_ensemble& de,
+                        const mda_ensemble* ens, const mda_geometry* geom, const dev_obs& o,
+                        const dev_hop& h) {
+  hipStream_t s = c->stream;
+  local_obs L;
+  const int k = de.k;
+  const int64_t p = o.p, n = de.n, nx = ens->nx, ny = ens->ny;
+  const int64_t ncol = nx * ny;
+  const bool gd = geom->on_device != 0;
+  const double* ca = L.ca = to_device(c, "geom_a", geom->col_a, ncol, gd);
+  const double* cb = L.cb = to_device(c, "geom_b", geom->col_b, ncol, gd);
+  L.cc = to_device(c, "geom_c", geom->col_c, ncol, gd);
+  MDA_REQUIRE(geom->coord == MDA_COORD_GRID || (ca && cb),
+              "GEOGRAPHIC / CARTESIAN geometry needs column coordinates");
+  L.rho_col = to_device(c, "rho_col", prm->rho_column, ncol, prm->rho_on_device != 0);
+  MDA_HIP(hipEventRecord(c->ev[1], s));
+
+  // 1. observation space
+  double* yp = c->get<double>("yp_om", (size_t)p * k);
+  double* dd = c->get<double>("d", p);
+  launch_obs_stats(s, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp, dd,
+                   nullptr);
+  MDA_HIP(hipEventRecord(c->ev[2], s));
+
+  // 2. bucket grid over the localisation coordinates
+  bucket_grid bg{};
+  bg.coord = geom->coord;
+  const double* rcoord;
+  const double* ccoord;
+  if (geom->coord == MDA_COORD_GEOGRAPHIC) {
+    double ddeg = prm->radius / 6371.0 * 180.0 / 3.14159265358979323846;
+    double step = std::min(10.0, std::max(0.05, ddeg / 2.0));
+    // bucket sizes divide the latitude and longitude ranges exactly
+    bg.nrow = (int)std::ceil(180.0 / step);
+    bg.ncolb = (int)std::ceil(360.0 / step);
+    bg.a0 = -90.0; bg.b0 = 0.0; bg.da = 180.0 / bg.nrow; bg.db = 360.0 / bg.ncolb;
+    bg.wrap = 1;
+    rcoord = o.a; ccoord = o.b;
+  } else {
+    double xmin = 0, xmax = (double)(nx - 1), ymin = 0, ymax = (double)(ny - 1);
+    if (geom->col_a) {
+      std::vector<double> hx(ncol), hy(ncol);
+      d2h(c, hx.data(), ca, sizeof(double) * ncol);
+      d2h(c, hy.data(), cb, sizeof(double) * ncol);
+      MDA_HIP(hipStreamSynchronize(s));
+      auto mx = std::minmax_element(hx.begin(), hx.end());
+      auto my = std::minmax_element(hy.begin(), hy.end());
+      xmin = *mx.first; xmax = *mx.second; ymin = *my.first; ymax = *my.second;
+    }
+    double r = std::min(prm->radius, 1e15);
+    double x0 = xmin - r, x1 = xmax + r, y0 = ymin - r, y1 = ymax + r;
+    double step = std::max(r / 2.0, std::max(x1 - x0, y1 - y0) / 2048.0);
+    bg.a0 = y0; bg.b0 = x0; bg.da = step; bg.db = step; bg.wrap = 0;
+    bg.nrow = std::max(1, (int)std::ceil((y1 - y0) / step) + 1);
+    bg.ncolb = std::max(1, (int)std::ceil((x1 - x0) / step) + 1);
+    rcoord = o.b; ccoord = o.a;
+  }
+  const int nbuckets = bg.nrow * bg.ncolb;
+  MDA_REQUIRE(nbuckets > 0 && nbuckets < (1 << 24), "bucket grid too large");
+  int key_bits = 1;
+  while ((1 << key_bits) < nbuckets) ++key_bits;
+  uint32_t* keys = c->get<uint32_t>("bk_keys", p);
+  uint32_t* vals = c->get<uint32_t>("bk_vals", p);
+  uint32_t* keys2 = c->get<uint32_t>("bk_keys2", p);
+  uint32_t* vals2 = c->get<uint32_t>("bk_vals2", p);
+  int64_t hist_cap = ((p + 255) / 256) * 256;
+  uint32_t* hist = c->get<uint32_t>("bk_hist", hist_cap);
+  int* bstart = c->get<int>("bk_start", nbuckets + 1);
+  launch_bucket_keys(s, rcoord, ccoord, p, bg, keys, vals);
+  radix_sort_pairs(s, keys, vals, keys2, vals2, p, key_bits, hist, hist_cap);
+  launch_bucket_ranges(s, keys, p, nbuckets, bstart);
+  double* s_yp = c->get<double>("s_yp", (size_t)p * k);
+  double* s_d = c->get<double>("s_d", p);
+  double* s_r = c->get<double>("s_rinv", p);
+  double* s_a = c->get<double>("s_a", p);
+  double* s_b = c->get<double>("s_b", p);
+  double* s_c = c->get<double>("s_c", p);
+  launch_gather_obs(s, vals, p, k, yp, dd, o.var, o.a, o.b, o.c, prm->rweight, s_yp, s_d, s_r,
+                    s_a, s_b, s_c);
+  float* s_y32 = nullptr;
+  double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
+  if (prm->precision != MDA_PREC_FP64) {
+    const bool geo = geom->coord == MDA_COORD_GEOGRAPHIC;
+    s_y32 = c->get<float>("s_y32", (size_t)p * k);
+    if (geo) {
+      s_ux = c->get<double>("s_ux", p);
+      s_uy = c->get<double>("s_uy", p);
+      s_uz = c->get<double>("s_uz", p);
+    }
+    launch_mixed_obs(s, p, k, s_yp, s_a, s_b, geo ? 1 : 0, s_y32, s_ux, s_uy, s_uz);
+  }
+  MDA_HIP(hipEventRecord(c->ev[3], s));
+  L.bg = bg; L.bstart = bstart;
+  L.s_yp = s_yp; L.s_d = s_d; L.s_r = s_r; L.s_a = s_a; L.s_b = s_b; L.s_c = s_c;
+  L.s_y32 = s_y32; L.s_ux = s_ux; L.s_uy = s_uy; L.s_uz = s_uz;
+  return L;
+}
+
 int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens,
                       const mda_geometry* geom, const mda_obs* obs, const mda_hop* hop,
                       mda_letkf_diag* diag) {
@@ -420,91 +532,16 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     const int64_t p = o.p, n = de.n, nx = ens->nx, ny = ens->ny, nlev = ens->nlev;
     const int64_t ncol = nx * ny;
     MDA_REQUIRE(k <= 64, "LETKF supports k <= 64 in this build");
-    const bool gd = geom->on_device != 0;
-    const double* ca = to_device(c, "geom_a", geom->col_a, ncol, gd);
-    const double* cb = to_device(c, "geom_b", geom->col_b, ncol, gd);
-    const double* cc = to_device(c, "geom_c", geom->col_c, ncol, gd);
-    MDA_REQUIRE(geom->coord == MDA_COORD_GRID || (ca && cb),
-                "GEOGRAPHIC / CARTESIAN geometry needs column coordinates");
-    const double* rho_col =
-        to_device(c, "rho_col", prm->rho_column, ncol, prm->rho_on_device != 0);
-    MDA_HIP(hipEventRecord(c->ev[1], s));
-
-    // 1. observation space
-    double* yp = c->get<double>("yp_om", (size_t)p * k);
-    double* dd = c->get<double>("d", p);
-    launch_obs_stats(s, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp, dd,
-                     nullptr);
-    MDA_HIP(hipEventRecord(c->ev[2], s));
-
-    // 2. bucket grid over the localisation coordinates
-    bucket_grid bg{};
-    bg.coord = geom->coord;
-    const double* rcoord;
-    const double* ccoord;
-    if (geom->coord == MDA_COORD_GEOGRAPHIC) {
-      double ddeg = prm->radius / 6371.0 * 180.0 / 3.14159265358979323846;
-      double step = std::min(10.0, std::max(0.05, ddeg / 2.0));
-      // bucket sizes divide the latitude and longitude ranges exactly
-      bg.nrow = (int)std::ceil(180.0 / step);
-      bg.ncolb = (int)std::ceil(360.0 / step);
-      bg.a0 = -90.0; bg.b0 = 0.0; bg.da = 180.0 / bg.nrow; bg.db = 360.0 / bg.ncolb;
-      bg.wrap = 1;
-      rcoord = o.a; ccoord = o.b;
-    } else {
-      double xmin = 0, xmax = (double)(nx - 1), ymin = 0, ymax = (double)(ny - 1);
-      if (geom->col_a) {
-        std::vector<double> hx(ncol), hy(ncol);
-        d2h(c, hx.data(), ca, sizeof(double) * ncol);
-        d2h(c, hy.data(), cb, sizeof(double) * ncol);
-        MDA_HIP(hipStreamSynchronize(s));
-        auto mx = std::minmax_element(hx.begin(), hx.end());
-        auto my = std::minmax_element(hy.begin(), hy.end());
-        xmin = *mx.first; xmax = *mx.second; ymin = *my.first; ymax = *my.second;
-      }
-      double r = std::min(prm->radius, 1e15);
-      double x0 = xmin - r, x1 = xmax + r, y0 = ymin - r, y1 = ymax + r;
-      double step = std::max(r / 2.0, std::max(x1 - x0, y1 - y0) / 2048.0);
-      bg.a0 = y0; bg.b0 = x0; bg.da = step; bg.db = step; bg.wrap = 0;
-      bg.nrow = std::max(1, (int)std::ceil((y1 - y0) / step) + 1);
-      bg.ncolb = std::max(1, (int)std::ceil((x1 - x0) / step) + 1);
-      rcoord = o.b; ccoord = o.a;
-    }
-    const int nbuckets = bg.nrow * bg.ncolb;
-    MDA_REQUIRE(nbuckets > 0 && nbuckets < (1 << 24), "bucket grid too large");
-    int key_bits = 1;
-    while ((1 << key_bits) < nbuckets) ++key_bits;
-    uint32_t* keys = c->get<uint32_t>("bk_keys", p);
-    uint32_t* vals = c->get<uint32_t>("bk_vals", p);
-    uint32_t* keys2 = c->get<uint32_t>("bk_keys2", p);
-    uint32_t* vals2 = c->get<uint32_t>("bk_vals2", p);
-    int64_t hist_cap = ((p + 255) / 256) * 256;
-    uint32_t* hist = c->get<uint32_t>("bk_hist", hist_cap);
-    int* bstart = c->get<int>("bk_start", nbuckets + 1);
-    launch_bucket_keys(s, rcoord, ccoord, p, bg, keys, vals);
-    radix_sort_pairs(s, keys, vals, keys2, vals2, p, key_bits, hist, hist_cap);
-    launch_bucket_ranges(s, keys, p, nbuckets, bstart);
-    double* s_yp = c->get<double>("s_yp", (size_t)p * k);
-    double* s_d = c->get<double>("s_d", p);
-    double* s_r = c->get<double>("s_rinv", p);
-    double* s_a = c->get<double>("s_a", p);
-    double* s_b = c->get<double>("s_b", p);
-    double* s_c = c->get<double>("s_c", p);
-    launch_gather_obs(s, vals, p, k, yp, dd, o.var, o.a, o.b, o.c, prm->rweight, s_yp, s_d, s_r,
-                      s_a, s_b, s_c);
-    float* s_y32 = nullptr;
-    double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
-    if (prm->precision != MDA_PREC_FP64) {
-      const bool geo = geom->coord == MDA_COORD_GEOGRAPHIC;
-      s_y32 = c->get<float>("s_y32", (size_t)p * k);
-      if (geo) {
-        s_ux = c->get<double>("s_ux", p);
-        s_uy = c->get<double>("s_uy", p);
-        s_uz = c->get<double>("s_uz", p);
-      }
-      launch_mixed_obs(s, p, k, s_yp, s_a, s_b, geo ? 1 : 0, s_y32, s_ux, s_uy, s_uz);
-    }
-    MDA_HIP(hipEventRecord(c->ev[3], s));
+    const local_obs L = prepare_local(c, prm, de, ens, geom, o, h);
+    const double* ca = L.ca;
+    const double* cb = L.cb;
+    const double* cc = L.cc;
+    const double* rho_col = L.rho_col;
+    const bucket_grid bg = L.bg;
+    const int* bstart = L.bstart;
+    double *s_yp = L.s_yp, *s_d = L.s_d, *s_r = L.s_r, *s_a = L.s_a, *s_b = L.s_b, *s_c = L.s_c;
+    float* s_y32 = L.s_y32;
+    double *s_ux = L.s_ux, *s_uy = L.s_uy, *s_uz = L.s_uz;
 
     // 3. per-column Gram
     double* C = c->get<double>("col_C", (size_t)ncol * k * k);
@@ -648,6 +685,106 @@ int mda_ensemble_metrics(mda_ctx* c, const mda_ensemble* ens, const double* trut
                        std::sqrt((nn * a[5] - a[2] * a[2]) * (nn * a[6] - a[3] * a[3]));
     out->avg_spread = a[7] / nn;
     out->crps = a[8] / nn;
+  });
+}
+
+
+// ---------------------------------------------------------------------------
+// learned inflation
+// ---------------------------------------------------------------------------
+int mda_l96_forecast(mda_ctx* c, double* X, int64_t nstates, int64_t N, double forcing,
+                     double dt, int32_t nsteps, int32_t scheme, int32_t on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(X != nullptr, "X is null");
+    MDA_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    MDA_REQUIRE(scheme == 0 || scheme == 1, "scheme is 0 (RK4) or 1 (Euler)");
+    MDA_REQUIRE(std::isfinite(dt) && std::isfinite(forcing), "dt and forcing must be finite");
+    double* d = X;
+    if (!on_device) {
+      d = c->get<double>("l96_states", (size_t)nstates * N);
+      h2d(c, d, X, sizeof(double) * nstates * N);
+    }
+    launch_l96(c->stream, d, nstates, N, forcing, dt, nsteps, scheme);
+    if (!on_device) d2h(c, X, d, sizeof(double) * nstates * N);
+    MDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mda_letkf_inflation_features(mda_ctx* c, const mda_letkf_params* prm,
+                                 const mda_ensemble* ens, const mda_geometry* geom,
+                                 const mda_obs* obs, const mda_hop* hop, const double* rho_prev,
+                                 float* features, int32_t features_on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(prm && geom && features, "params, geometry and features are required");
+    MDA_REQUIRE(prm->radius > 0.0, "radius must be > 0");
+    MDA_REQUIRE(geom->coord >= 0 && geom->coord <= 2, "unknown coordinate system");
+    hipStream_t s = c->stream;
+    mda_letkf_params fp = *prm;
+    fp.precision = MDA_PREC_FP64;
+    fp.rho_column = nullptr;
+    dev_ensemble de = stage_ensemble(c, ens);
+    dev_obs o = stage_obs(c, obs, true);
+    dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
+    const int64_t ncol = ens->nx * ens->ny;
+    const local_obs L = prepare_local(c, &fp, de, ens, geom, o, h);
+    letkf_gram_args ga{};
+    ga.k = de.k; ga.ncol = ncol; ga.nx = ens->nx; ga.coord = geom->coord;
+    ga.col_a = L.ca; ga.col_b = L.cb; ga.col_c = L.cc;
+    ga.grid = L.bg; ga.bstart = L.bstart;
+    ga.yp = L.s_yp; ga.d = L.s_d; ga.rinv = L.s_r; ga.oa = L.s_a; ga.ob = L.s_b; ga.oc = L.s_c;
+    ga.radius = prm->radius; ga.localization = prm->localization;
+    double* sp = c->get<double>("feat_sp", o.p);
+    launch_obs_spread(s, L.s_yp, o.p, de.k, sp);
+    const double* rp = to_device(c, "feat_rho_prev", rho_prev, ncol, prm->rho_on_device != 0);
+    float* f = features_on_device ? features : c->get<float>("feat_out", (size_t)ncol * 8);
+    launch_letkf_features(s, ga, sp, de.X, ens->nlev, rp, f);
+    if (!features_on_device) d2h(c, features, f, sizeof(float) * ncol * 8);
+    MDA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int mda_mlp_forward(mda_ctx* c, const float* params, const float* features, int64_t m,
+                    float* rho, int32_t on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(params && features && rho, "params, features and rho are required");
+    MDA_REQUIRE(m >= 0, "m must be >= 0");
+    const bool od = on_device != 0;
+    const float* P = to_device(c, "mlp_params_in", params, MDA_MLP_NPARAM, od);
+    const float* F = to_device(c, "mlp_feat_in", features, (size_t)m * MDA_MLP_NIN, od);
+    float* R = od ? rho : c->get<float>("mlp_rho_out", m);
+    launch_mlp_forward(c->stream, P, F, m, R);
+    if (!od) d2h(c, rho, R, sizeof(float) * m);
+    MDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mda_mlp_loss_grad(mda_ctx* c, const float* params, const float* features,
+                      const float* target, int64_t m, double* loss, float* grad,
+                      int32_t on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(params && features && target && loss && grad, "null argument");
+    MDA_REQUIRE(m > 0, "m must be > 0");
+    const bool od = on_device != 0;
+    const float* P = to_device(c, "mlp_params_in", params, MDA_MLP_NPARAM, od);
+    const float* F = to_device(c, "mlp_feat_in", features, (size_t)m * MDA_MLP_NIN, od);
+    const float* T = to_device(c, "mlp_target_in", target, m, od);
+    float* G = od ? grad : c->get<float>("mlp_grad_out", MDA_MLP_NPARAM);
+    double* Ld = c->get<double>("mlp_loss", 1);
+    launch_mlp_loss_grad(c, c->stream, P, F, T, m, G, Ld);
+    d2h(c, loss, Ld, sizeof(double));
+    if (!od) d2h(c, grad, G, sizeof(float) * MDA_MLP_NPARAM);
+  });
+}
+
+int mda_mlp_adam_step(mda_ctx* c, float* params, float* m1, float* m2, const float* grad,
+                      int64_t step, float lr, float beta1, float beta2, float eps,
+                      int32_t on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(params && m1 && m2 && grad, "null argument");
+    MDA_REQUIRE(on_device != 0, "mda_mlp_adam_step works on device arrays");
+    MDA_REQUIRE(lr > 0.f && beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
+                "bad Adam hyper-parameters");
+    launch_mlp_adam(c->stream, params, m1, m2, grad, step, lr, beta1, beta2, eps);
   });
 }
 

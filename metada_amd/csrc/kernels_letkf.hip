@@ -738,6 +738,128 @@ void launch_mixed_obs(hipStream_t s, int64_t p, int k, const double* s_yp, const
   MDA_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// Local innovation and spread statistics of each column, the inputs of the
+// learned inflation.  Same candidate search and weights as the FP64 Gram
+// (w_o = localisation x 1/var_o); per column
+//   Sw = sum w,  Sd2 = sum w d^2,  Sd = sum w d,  Ssp = sum w s_o
+// with s_o = |y'_o|^2 / (k-1) the background variance of observation o, and
+// the column's background variance v (member variance averaged over levels).
+// Features (FP32): log(Sd2/Sw), log(Ssp/Sw), log(Sd2/(Ssp+Sw)),
+// Sd/sqrt(Sw Sd2), log(Sw), log(1+n_local), log(v), rho_prev - 1.
+// ---------------------------------------------------------------------------
+__global__ void k_obs_spread(const double* __restrict__ s_yp, int64_t p, int k,
+                             double* __restrict__ sp) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < p;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int m = 0; m < k; ++m) acc += s_yp[(size_t)t * k + m] * s_yp[(size_t)t * k + m];
+    sp[t] = acc / (double)(k - 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_letkf_features(letkf_gram_args a,
+                                                        const double* __restrict__ sp,
+                                                        const double* __restrict__ X,
+                                                        int64_t nlev,
+                                                        const double* __restrict__ rho_prev,
+                                                        float* __restrict__ feat) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = a.k;
+  const bucket_grid& G = a.grid;
+  const double cgc = a.radius / 2.0;
+  const int64_t n = a.ncol * nlev;
+  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
+    const col_window cw = column_window(a, g);
+    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
+    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
+    double Sw = 0.0, Sd2 = 0.0, Sd = 0.0, Ssp = 0.0;
+    int nl = 0;
+    for (int row = cw.row0; row <= cw.row1; ++row) {
+      for (int sg = 0; sg < cw.sl.n; ++sg) {
+        const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
+        const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
+        for (int64_t t = s0 + lane; t < s1; t += 64) {
+          double dist;
+          if (a.coord == MDA_COORD_GEOGRAPHIC) {
+            dist = haversine_km(cw.ca, cw.cb, a.oa[t], a.ob[t]);
+          } else {
+            const double dx = cw.ca - a.oa[t], dy = cw.cb - a.ob[t];
+            double d2 = dx * dx + dy * dy;
+            if (a.coord == MDA_COORD_CARTESIAN) {
+              const double dz = cw.cc - a.oc[t];
+              d2 += dz * dz;
+            }
+            dist = sqrt(d2);
+          }
+          if (dist <= a.radius) {
+            ++nl;
+            const double w = fmax((a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / cgc)
+                                                                           : 1.0) *
+                                      a.rinv[t],
+                                  0.0);
+            const double d = a.d[t];
+            Sw += w;
+            Sd2 += w * d * d;
+            Sd += w * d;
+            Ssp += w * sp[t];
+          }
+        }
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      Sw += __shfl_xor(Sw, off);
+      Sd2 += __shfl_xor(Sd2, off);
+      Sd += __shfl_xor(Sd, off);
+      Ssp += __shfl_xor(Ssp, off);
+      nl += __shfl_xor(nl, off);
+    }
+    // background variance of the column, averaged over levels
+    double var = 0.0;
+    for (int64_t l = 0; l < nlev; ++l) {
+      double sx = 0.0, sxx = 0.0;
+      for (int m = lane; m < k; m += 64) {
+        const double v = X[(size_t)m * n + l * a.ncol + g];
+        sx += v;
+        sxx += v * v;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sxx += __shfl_xor(sxx, off);
+      }
+      var += fmax(sxx - sx * sx / k, 0.0) / (double)(k - 1);
+    }
+    var /= (double)nlev;
+    if (lane == 0) {
+      const double tiny = 1e-300;
+      float* f = feat + (size_t)g * 8;
+      const bool has = Sw > 0.0;
+      f[0] = has ? (float)log(Sd2 / Sw + tiny) : 0.f;
+      f[1] = has ? (float)log(Ssp / Sw + tiny) : 0.f;
+      f[2] = has ? (float)log(Sd2 / (Ssp + Sw) + tiny) : 0.f;
+      f[3] = has && Sd2 > 0.0 ? (float)(Sd / sqrt(Sw * Sd2)) : 0.f;
+      f[4] = has ? (float)log(Sw) : 0.f;
+      f[5] = (float)log1p((double)nl);
+      f[6] = (float)log(var + tiny);
+      f[7] = rho_prev ? (float)(rho_prev[g] - 1.0) : 0.f;
+    }
+  }
+}
+
+void launch_letkf_features(hipStream_t s, const letkf_gram_args& a, double* sp_work,
+                           const double* X, int64_t nlev, const double* rho_prev, float* feat) {
+  hipLaunchKernelGGL(k_letkf_features, dim3(grid_for(a.ncol, 4, 262144)), dim3(256), 0, s, a,
+                     sp_work, X, nlev, rho_prev, feat);
+  MDA_LAUNCH_CHECK();
+}
+
+void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, double* sp) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_obs_spread, dim3(grid_for(p, 256, 65536)), dim3(256), 0, s, s_yp, p, k,
+                     sp);
+  MDA_LAUNCH_CHECK();
+}
+
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   unsigned nb = (unsigned)((a.ncol + 3) / 4);
   if (nb > 262144) nb = 262144;

@@ -212,6 +212,21 @@ struct letkf_gram_args {
   int* nloc;    // out [ncol]
 };
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
+// learned-inflation inputs: per-observation background variance of the
+// sorted Y', then the 8 local statistics of every column
+void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, double* sp);
+void launch_letkf_features(hipStream_t s, const letkf_gram_args& a, double* sp,
+                           const double* X, int64_t nlev, const double* rho_prev, float* feat);
+// Lorenz-96 forecast of nstates states of size N in place (scheme 0 RK4, 1 Euler)
+void launch_l96(hipStream_t s, double* X, int64_t nstates, int64_t N, double F, double dt,
+                int nsteps, int scheme);
+// learned inflation network
+void launch_mlp_forward(hipStream_t s, const float* params, const float* feat, int64_t m,
+                        float* rho);
+void launch_mlp_loss_grad(mda_ctx* c, hipStream_t s, const float* params, const float* feat,
+                          const float* target, int64_t m, float* grad, double* loss);
+void launch_mlp_adam(hipStream_t s, float* params, float* m1, float* m2, const float* grad,
+                     int64_t step, float lr, float b1, float b2, float eps);
 struct letkf_solve_args {
   int k;
   int64_t ncol;
