@@ -108,3 +108,19 @@ def test_trilinear_reproduces_affine_fields():
     hx = H.apply_host(f.reshape(1, -1))[0]
     np.testing.assert_allclose(hx, 2 * lat + 0.1 * lon + 3 * lev, rtol=1e-12, atol=1e-10)
     assert np.allclose(np.diff(H.row_ptr), 8)
+
+
+def test_cpp_adapter_compiles_against_reference_shaped_types():
+    """include/metada_amd.hpp instantiated with stand-ins that have the
+    reference adapter API's member names (tests/cpp/adapter_mock.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "cpp", "adapter_mock.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
