@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--skip-plain", action="store_true",
                     help="skip the FP64 Jacobi (plain) path")
+    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                    help="only copy this many GiB device-to-device once (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--value-path-only", action="store_true",
                     help="run only the --precision path (for profiler runs): no oracle, "
                          "no FP64 reference, no error report")
@@ -63,6 +66,15 @@ def main():
     import numpy as np
     import torch
     log(f"torch {torch.__version__}, device {torch.cuda.get_device_name(0)}")
+    if a.copy_calibration_gib > 0:
+        nel = int(a.copy_calibration_gib * (1 << 30)) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nel * 8}), flush=True)
+        return 0
     import metada_amd as mda
     from metada_amd import synthetic
     from metada_amd import _lib as L
@@ -260,8 +272,18 @@ def main():
     }
     dom = max(cand, key=lambda kk: cand[kk][5])
     kern, b, ach, pk, unit, _ = cand[dom]
+    # HBM bytes per launch of that kernel from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs (scaled by a copy of known size), for the same workload
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_letkf_{a.precision}_{a.nlon}x{a.nlat}x{nlev}_k{k}.json")
+    if os.path.exists(tf) and p == 1_000_000:
+        with open(tf) as fh:
+            tj = json.load(fh)
+        for name, v in tj["kernels"].items():
+            if name.split("::")[-1].split("<")[0] == kern:
+                traffic = v["traffic"]
     roof = {"kernel": kern, "bound": b, "achieved": ach, "peak": pk, "unit": unit,
-            "frac": ach / pk, "traffic": None, "ms_per_launch": cand[dom][5]}
+            "frac": ach / pk, "traffic": traffic, "ms_per_launch": cand[dom][5]}
     ratio_plain = (res["fp64_jacobi"]["ms_median"] / vres["ms_median"]
                    if "fp64_jacobi" in res else None)
     out = {
