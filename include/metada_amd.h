@@ -205,6 +205,35 @@ int mda_letkf_analyse(mda_ctx* ctx, const mda_letkf_params* prm, mda_ensemble* e
                       const mda_geometry* geom, const mda_obs* obs, const mda_hop* hop,
                       mda_letkf_diag* diag);
 
+/* Batched LETKF (the batching form): `nb` independent analyses -- any mix of
+ * ensemble sizes, precisions and observation sets, e.g. k = 32 and 64 over
+ * clustered observations -- given as arrays of nb structs.  Flags:
+ *   MDA_BATCH_GRAPH     capture all analyses as concurrent branches of one
+ *                       hipGraph on the first call and replay it when the
+ *                       same buffers and parameters come again (inputs must
+ *                       then be device-resident or are re-staged each call);
+ *   MDA_BATCH_QUEUE     per-column Gram in work-queue mode: persistent waves
+ *                       take columns in descending cost (observations in reach),
+ *                       which balances clustered observation counts;
+ *   MDA_BATCH_AUTOTUNE  on the first call of a workload (k, precision, columns,
+ *                       observations) time candidate Gram launch shapes and keep
+ *                       the fastest for the context.
+ * Ensembles, observations and H must be device-resident (on_device).
+ * Each analysis gives the same result as mda_letkf_analyse.  No counterpart in
+ * the reference (its algorithms run one analysis at a time, LETKF.hpp:63). */
+enum { MDA_BATCH_GRAPH = 1, MDA_BATCH_QUEUE = 2, MDA_BATCH_AUTOTUNE = 4 };
+typedef struct {
+  double ms_total;          /* the whole batch (HIP events around the launch) */
+  int32_t graph_replayed;   /* 1 if an instantiated graph was replayed */
+  int32_t n_graphs;         /* graphs held by the context */
+  int32_t gram_blocks[16];  /* Gram launch blocks per analysis (0 = one wave per column) */
+  int32_t gram_queue[16];   /* 1 if the Gram ran in work-queue mode */
+} mda_batch_info;
+int mda_letkf_analyse_batch(mda_ctx* ctx, int32_t nb, const mda_letkf_params* prm,
+                            mda_ensemble* ens, const mda_geometry* geom, const mda_obs* obs,
+                            const mda_hop* hop, mda_letkf_diag* diag, int32_t flags,
+                            mda_batch_info* info);
+
 /* Per-column k x k transforms T (X_a = xbar + X' T) of the last LETKF call,
  * for the listed columns, row-major [ncols][k][k] (host).  Diagnostics and
  * parity tests; valid until the next call on the context. */

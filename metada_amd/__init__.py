@@ -7,8 +7,9 @@ analysis interface on top of it.  Importing it does not touch the GPU.
 from . import _lib
 from .analysis import (ColumnGeometry, Config, ConfigValue, Context, EnKF, Ensemble, ETKF,
                        LETKF, ObsOperator, Observation, default_context, idw4_stencil,
-                       metrics, read_simple_state)
+                       letkf_batch, metrics, read_simple_state)
 
 __all__ = ["ColumnGeometry", "Config", "ConfigValue", "Context", "EnKF", "Ensemble", "ETKF",
-           "LETKF", "ObsOperator", "Observation", "default_context", "idw4_stencil", "metrics",
+           "LETKF", "ObsOperator", "Observation", "default_context", "idw4_stencil", "letkf_batch",
+           "metrics",
            "read_simple_state", "_lib"]

@@ -76,6 +76,14 @@ class LetkfDiag(C.Structure):
                 ("ms_eig_vectors", _d), ("ms_solve_jacobi", _d)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("ms_total", _d), ("graph_replayed", _i32), ("n_graphs", _i32),
+                ("gram_blocks", _i32 * 16), ("gram_queue", _i32 * 16)]
+
+
+BATCH_GRAPH, BATCH_QUEUE, BATCH_AUTOTUNE = 1, 2, 4
+
+
 class Metrics(C.Structure):
     _fields_ = [("rmse", _d), ("bias", _d), ("correlation", _d), ("crps", _d),
                 ("avg_spread", _d)]
@@ -104,6 +112,9 @@ SIGNATURES = {
     "mda_letkf_analyse": (C.c_int, [C.c_void_p, C.POINTER(LetkfParams), C.POINTER(Ensemble),
                                     C.POINTER(Geometry), C.POINTER(Obs), C.POINTER(Hop),
                                     C.POINTER(LetkfDiag)]),
+    "mda_letkf_analyse_batch": (C.c_int, [C.c_void_p, _i32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, _i32,
+                                          C.POINTER(BatchInfo)]),
     "mda_letkf_last_transforms": (C.c_int, [C.c_void_p, C.c_void_p, _i64, C.c_void_p]),
     "mda_letkf_last_local_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mda_letkf_last_column_flags": (C.c_int, [C.c_void_p, C.c_void_p]),

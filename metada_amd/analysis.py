@@ -607,3 +607,33 @@ def read_simple_state(path: str) -> np.ndarray:
         raise RuntimeError(f"Cannot open state file: {path}")
     with open(path) as f:
         return np.array([float(v) for v in f.read().split()])
+
+
+def letkf_batch(filters, flags=L.BATCH_GRAPH | L.BATCH_QUEUE | L.BATCH_AUTOTUNE):
+    """Runs independent LETKF analyses (e.g. different ensemble sizes) as one
+    batch (mda_letkf_analyse_batch): concurrent branches of one hipGraph,
+    work-queue Gram, autotuned launch.  Inputs must be device-resident.
+    Returns the per-analysis diagnostics and the batch info."""
+    ctx = filters[0].ctx
+    n = len(filters)
+    P = (L.LetkfParams * n)(*[f.params for f in filters])
+    E, G, O, H = (L.Ensemble * n)(), (L.Geometry * n)(), (L.Obs * n)(), (L.Hop * n)()
+    keep = []
+    for i, f in enumerate(filters):
+        assert f.ctx is ctx, "all analyses of a batch share one context"
+        es, ek = f.ensemble.struct()
+        gs, gk = f.geometry.struct()
+        os_, ok = _obs_struct(f.obs)
+        hs, hk = f.obs_op.struct()
+        E[i], G[i], O[i], H[i] = es, gs, os_, hs
+        keep += [ek, gk, ok, hk]
+    D = (L.LetkfDiag * n)()
+    info = L.BatchInfo()
+    ctx.check(ctx.lib.mda_letkf_analyse_batch(ctx.h, n, P, E, G, O, H, D, flags, C.byref(info)))
+    del keep
+    diags = [{fld: getattr(D[i], fld) for fld, _ in L.LetkfDiag._fields_} for i in range(n)]
+    for f, d in zip(filters, diags):
+        f.diag = d
+    return diags, {"ms_total": info.ms_total, "graph_replayed": info.graph_replayed,
+                   "n_graphs": info.n_graphs, "gram_blocks": list(info.gram_blocks[:n]),
+                   "gram_queue": list(info.gram_queue[:n])}

@@ -185,6 +185,14 @@ int mda_ctx_destroy(mda_ctx* c) {
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& kv : c->graphs)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
+  for (auto st : c->side) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  for (auto e : c->fj)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipSetDevice(prev);
   delete c;
@@ -399,49 +407,83 @@ void mda_letkf_params_refcompat(mda_letkf_params* p, double inflation, double ra
   p->precision = MDA_PREC_FP64;
 }
 
-// Geometry staging, observation-space statistics (HX, Y', d), the bucket
-// index over the localisation coordinates and the gather of the observation
-// data into bucket order: everything before the per-column work of the
-// LETKF (LETKF.hpp:152-211).  Records ev[1..3].
-struct local_obs {
-  const double *ca = nullptr, *cb = nullptr, *cc = nullptr, *rho_col = nullptr;
+// ---------------------------------------------------------------------------
+// LETKF pipeline (LETKF.hpp:63-243), in three parts:
+//   plan     host-side validation, staging of host inputs, the bucket grid and
+//            every workspace buffer (may synchronise; nothing is enqueued);
+//   enqueue  the kernels only, on a given stream (no host synchronisation or
+//            allocation, so it can be captured into a hipGraph);
+//   finish   device statistics to the host and the diagnostics.
+// Workspace names carry a prefix so the analyses of one batch do not share
+// buffers.
+// ---------------------------------------------------------------------------
+struct letkf_plan {
+  const mda_letkf_params* prm = nullptr;
+  mda_ensemble* ens = nullptr;
+  const mda_geometry* geom = nullptr;
+  dev_ensemble de;
+  dev_obs o;
+  dev_hop h;
+  int k = 0;
+  int64_t p = 0, n = 0, nx = 0, ny = 0, nlev = 0, ncol = 0;
+  const double *rcoord = nullptr, *ccoord = nullptr;
   bucket_grid bg{};
-  const int* bstart = nullptr;
-  double *s_yp = nullptr, *s_d = nullptr, *s_r = nullptr, *s_a = nullptr, *s_b = nullptr,
-         *s_c = nullptr;
-  float* s_y32 = nullptr;
-  double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
+  int nbuckets = 0, key_bits = 1;
+  int64_t hist_cap = 0;
+  double *yp = nullptr, *dd = nullptr;
+  uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *hist = nullptr;
+  int* bstart = nullptr;
+  int geo = 0;
+  letkf_gram_args ga{};
+  letkf_solve_args sa{};
+  int* sweeps = nullptr;
+  int64_t* stats = nullptr;
+  bool eig_path = false;
+  // work-queue Gram: cost keys, column order and the counter
+  bool queue = false;
+  uint32_t *ckeys = nullptr, *ccols = nullptr, *ckeys2 = nullptr, *ccols2 = nullptr,
+           *chist = nullptr, *qcount = nullptr;
+  int64_t chist_cap = 0;
 };
 
-static local_obs prepare_local(mda_ctx* c, const mda_letkf_params* prm, const dev_ensemble& de,
-                        const mda_ensemble* ens, const mda_geometry* geom, const dev_obs& o,
-                        const dev_hop& h) {
-  hipStream_t s = c->stream;
-  local_obs L;
-  const int k = de.k;
-  const int64_t p = o.p, n = de.n, nx = ens->nx, ny = ens->ny;
-  const int64_t ncol = nx * ny;
+static void check_letkf_params(const mda_letkf_params* prm, const mda_geometry* geom) {
+  MDA_REQUIRE(prm && geom, "params and geometry are required");
+  MDA_REQUIRE(std::isfinite(prm->inflation) && prm->inflation > 0.0, "inflation must be > 0");
+  MDA_REQUIRE(prm->radius > 0.0, "radius must be > 0");
+  MDA_REQUIRE(prm->precision == MDA_PREC_FP64 || prm->precision == MDA_PREC_FP32 ||
+                  prm->precision == MDA_PREC_BF16X3,
+              "precision must be FP64, FP32 or BF16X3");
+  MDA_REQUIRE(geom->coord >= 0 && geom->coord <= 2, "unknown coordinate system");
+}
+
+static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens,
+                             const mda_geometry* geom, const mda_obs* obs, const mda_hop* hop,
+                             const std::string& pre, bool queue) {
+  check_letkf_params(prm, geom);
+  letkf_plan P;
+  P.prm = prm; P.ens = ens; P.geom = geom;
+  P.de = stage_ensemble(c, ens);
+  P.o = stage_obs(c, obs, true);
+  P.h = stage_hop(c, hop, P.de.k, P.o.p, P.de.n);
+  const int k = P.k = P.de.k;
+  const int64_t p = P.p = P.o.p;
+  P.n = P.de.n; P.nx = ens->nx; P.ny = ens->ny; P.nlev = ens->nlev;
+  const int64_t ncol = P.ncol = P.nx * P.ny;
+  MDA_REQUIRE(k <= 64, "LETKF supports k <= 64 in this build");
   const bool gd = geom->on_device != 0;
-  const double* ca = L.ca = to_device(c, "geom_a", geom->col_a, ncol, gd);
-  const double* cb = L.cb = to_device(c, "geom_b", geom->col_b, ncol, gd);
-  L.cc = to_device(c, "geom_c", geom->col_c, ncol, gd);
+  const double* ca = to_device(c, pre + "geom_a", geom->col_a, ncol, gd);
+  const double* cb = to_device(c, pre + "geom_b", geom->col_b, ncol, gd);
+  const double* cc = to_device(c, pre + "geom_c", geom->col_c, ncol, gd);
   MDA_REQUIRE(geom->coord == MDA_COORD_GRID || (ca && cb),
               "GEOGRAPHIC / CARTESIAN geometry needs column coordinates");
-  L.rho_col = to_device(c, "rho_col", prm->rho_column, ncol, prm->rho_on_device != 0);
-  MDA_HIP(hipEventRecord(c->ev[1], s));
+  const double* rho_col =
+      to_device(c, pre + "rho_col", prm->rho_column, ncol, prm->rho_on_device != 0);
+  P.yp = c->get<double>(pre + "yp_om", (size_t)p * k);
+  P.dd = c->get<double>(pre + "d", p);
 
-  // 1. observation space
-  double* yp = c->get<double>("yp_om", (size_t)p * k);
-  double* dd = c->get<double>("d", p);
-  launch_obs_stats(s, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp, dd,
-                   nullptr);
-  MDA_HIP(hipEventRecord(c->ev[2], s));
-
-  // 2. bucket grid over the localisation coordinates
-  bucket_grid bg{};
+  // bucket grid over the localisation coordinates
+  bucket_grid& bg = P.bg;
   bg.coord = geom->coord;
-  const double* rcoord;
-  const double* ccoord;
   if (geom->coord == MDA_COORD_GEOGRAPHIC) {
     double ddeg = prm->radius / 6371.0 * 180.0 / 3.14159265358979323846;
     double step = std::min(10.0, std::max(0.05, ddeg / 2.0));
@@ -450,14 +492,13 @@ static local_obs prepare_local(mda_ctx* c, const mda_letkf_params* prm, const de
     bg.ncolb = (int)std::ceil(360.0 / step);
     bg.a0 = -90.0; bg.b0 = 0.0; bg.da = 180.0 / bg.nrow; bg.db = 360.0 / bg.ncolb;
     bg.wrap = 1;
-    rcoord = o.a; ccoord = o.b;
+    P.rcoord = P.o.a; P.ccoord = P.o.b;
   } else {
-    double xmin = 0, xmax = (double)(nx - 1), ymin = 0, ymax = (double)(ny - 1);
+    double xmin = 0, xmax = (double)(P.nx - 1), ymin = 0, ymax = (double)(P.ny - 1);
     if (geom->col_a) {
       std::vector<double> hx(ncol), hy(ncol);
       d2h(c, hx.data(), ca, sizeof(double) * ncol);
       d2h(c, hy.data(), cb, sizeof(double) * ncol);
-      MDA_HIP(hipStreamSynchronize(s));
       auto mx = std::minmax_element(hx.begin(), hx.end());
       auto my = std::minmax_element(hy.begin(), hy.end());
       xmin = *mx.first; xmax = *mx.second; ymin = *my.first; ymax = *my.second;
@@ -468,157 +509,176 @@ static local_obs prepare_local(mda_ctx* c, const mda_letkf_params* prm, const de
     bg.a0 = y0; bg.b0 = x0; bg.da = step; bg.db = step; bg.wrap = 0;
     bg.nrow = std::max(1, (int)std::ceil((y1 - y0) / step) + 1);
     bg.ncolb = std::max(1, (int)std::ceil((x1 - x0) / step) + 1);
-    rcoord = o.b; ccoord = o.a;
+    P.rcoord = P.o.b; P.ccoord = P.o.a;
   }
-  const int nbuckets = bg.nrow * bg.ncolb;
-  MDA_REQUIRE(nbuckets > 0 && nbuckets < (1 << 24), "bucket grid too large");
-  int key_bits = 1;
-  while ((1 << key_bits) < nbuckets) ++key_bits;
-  uint32_t* keys = c->get<uint32_t>("bk_keys", p);
-  uint32_t* vals = c->get<uint32_t>("bk_vals", p);
-  uint32_t* keys2 = c->get<uint32_t>("bk_keys2", p);
-  uint32_t* vals2 = c->get<uint32_t>("bk_vals2", p);
-  int64_t hist_cap = ((p + 255) / 256) * 256;
-  uint32_t* hist = c->get<uint32_t>("bk_hist", hist_cap);
-  int* bstart = c->get<int>("bk_start", nbuckets + 1);
-  launch_bucket_keys(s, rcoord, ccoord, p, bg, keys, vals);
-  radix_sort_pairs(s, keys, vals, keys2, vals2, p, key_bits, hist, hist_cap);
-  launch_bucket_ranges(s, keys, p, nbuckets, bstart);
-  double* s_yp = c->get<double>("s_yp", (size_t)p * k);
-  double* s_d = c->get<double>("s_d", p);
-  double* s_r = c->get<double>("s_rinv", p);
-  double* s_a = c->get<double>("s_a", p);
-  double* s_b = c->get<double>("s_b", p);
-  double* s_c = c->get<double>("s_c", p);
-  launch_gather_obs(s, vals, p, k, yp, dd, o.var, o.a, o.b, o.c, prm->rweight, s_yp, s_d, s_r,
-                    s_a, s_b, s_c);
-  float* s_y32 = nullptr;
-  double *s_ux = nullptr, *s_uy = nullptr, *s_uz = nullptr;
+  P.nbuckets = bg.nrow * bg.ncolb;
+  MDA_REQUIRE(P.nbuckets > 0 && P.nbuckets < (1 << 24), "bucket grid too large");
+  while ((1 << P.key_bits) < P.nbuckets) ++P.key_bits;
+  P.keys = c->get<uint32_t>(pre + "bk_keys", p);
+  P.vals = c->get<uint32_t>(pre + "bk_vals", p);
+  P.keys2 = c->get<uint32_t>(pre + "bk_keys2", p);
+  P.vals2 = c->get<uint32_t>(pre + "bk_vals2", p);
+  P.hist_cap = ((p + 255) / 256) * 256;
+  P.hist = c->get<uint32_t>(pre + "bk_hist", P.hist_cap);
+  P.bstart = c->get<int>(pre + "bk_start", P.nbuckets + 1);
+
+  letkf_gram_args& ga = P.ga;
+  ga.k = k; ga.ncol = ncol; ga.nx = P.nx; ga.coord = geom->coord;
+  ga.col_a = ca; ga.col_b = cb; ga.col_c = cc;
+  ga.grid = bg; ga.bstart = P.bstart;
+  ga.yp = c->get<double>(pre + "s_yp", (size_t)p * k);
+  ga.d = c->get<double>(pre + "s_d", p);
+  ga.rinv = c->get<double>(pre + "s_rinv", p);
+  ga.oa = c->get<double>(pre + "s_a", p);
+  ga.ob = c->get<double>(pre + "s_b", p);
+  ga.oc = c->get<double>(pre + "s_c", p);
+  ga.radius = prm->radius; ga.localization = prm->localization; ga.precision = prm->precision;
+  P.geo = geom->coord == MDA_COORD_GEOGRAPHIC;
   if (prm->precision != MDA_PREC_FP64) {
-    const bool geo = geom->coord == MDA_COORD_GEOGRAPHIC;
-    s_y32 = c->get<float>("s_y32", (size_t)p * k);
-    if (geo) {
-      s_ux = c->get<double>("s_ux", p);
-      s_uy = c->get<double>("s_uy", p);
-      s_uz = c->get<double>("s_uz", p);
+    ga.y32 = c->get<float>(pre + "s_y32", (size_t)p * k);
+    if (P.geo) {
+      ga.ux = c->get<double>(pre + "s_ux", p);
+      ga.uy = c->get<double>(pre + "s_uy", p);
+      ga.uz = c->get<double>(pre + "s_uz", p);
     }
-    launch_mixed_obs(s, p, k, s_yp, s_a, s_b, geo ? 1 : 0, s_y32, s_ux, s_uy, s_uz);
   }
-  MDA_HIP(hipEventRecord(c->ev[3], s));
-  L.bg = bg; L.bstart = bstart;
-  L.s_yp = s_yp; L.s_d = s_d; L.s_r = s_r; L.s_a = s_a; L.s_b = s_b; L.s_c = s_c;
-  L.s_y32 = s_y32; L.s_ux = s_ux; L.s_uy = s_uy; L.s_uz = s_uz;
-  return L;
+  ga.C = c->get<double>(pre + "col_C", (size_t)ncol * k * k);
+  ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
+  ga.nloc = c->get<int>(pre + "col_nloc", ncol);
+  P.queue = queue;
+  if (queue) {
+    P.ckeys = c->get<uint32_t>(pre + "cq_keys", ncol);
+    P.ccols = c->get<uint32_t>(pre + "cq_cols", ncol);
+    P.ckeys2 = c->get<uint32_t>(pre + "cq_keys2", ncol);
+    P.ccols2 = c->get<uint32_t>(pre + "cq_cols2", ncol);
+    P.chist_cap = ((ncol + 255) / 256) * 256;
+    P.chist = c->get<uint32_t>(pre + "cq_hist", P.chist_cap);
+    P.qcount = c->get<uint32_t>(pre + "cq_count", 1);
+  }
+
+  letkf_solve_args& sa = P.sa;
+  sa.k = k; sa.ncol = ncol; sa.C = ga.C; sa.b = ga.b; sa.nloc = ga.nloc;
+  sa.rho = prm->inflation; sa.rho_col = rho_col;
+  sa.inflation_form = prm->inflation_form; sa.sqrt_form = prm->sqrt_form;
+  sa.combine = prm->combine;
+  sa.T = c->get<double>(pre + "col_T", (size_t)ncol * k * k);
+  P.sweeps = sa.sweeps = c->get<int>(pre + "col_sweeps", ncol);
+  sa.status = c->get<int>(pre + "col_status", 1);
+  sa.de = c->get<double>(pre + "eig_de", (size_t)ncol * 2 * k);
+  sa.lam = c->get<double>(pre + "eig_lam", (size_t)ncol * k);
+  P.eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(k) &&
+               prm->eig_method == MDA_EIG_AUTO;
+  sa.U = P.eig_path ? c->get<double>(pre + "eig_U", (size_t)ncol * k * k) : nullptr;
+  sa.flag = c->get<int>(pre + "eig_flag", ncol);
+  sa.method = prm->eig_method;
+  // eigenvalue noise of the Gram relative to ||A||: FP32 products with FP32
+  // accumulation, or BF16x3 products (2^-16 relative) with FP32 accumulation
+  sa.null_rel = prm->precision == MDA_PREC_FP32 ? 0x1p-22
+                : prm->precision == MDA_PREC_BF16X3 ? 0x1p-17 : 0.0;
+  P.stats = c->get<int64_t>(pre + "col_stats", 9);
+  return P;
+}
+
+// Stages before the per-column work (LETKF.hpp:152-211): HX, Y', d, the
+// bucket index and the gathered observations.  Idempotent.
+static void enqueue_local(hipStream_t s, letkf_plan& P) {
+  const mda_letkf_params* prm = P.prm;
+  launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, P.h.hx, P.o.yo,
+                   nullptr, P.yp, P.dd, nullptr);
+}
+
+static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
+  letkf_gram_args& ga = P.ga;
+  launch_bucket_keys(s, P.rcoord, P.ccoord, P.p, P.bg, P.keys, P.vals);
+  radix_sort_pairs(s, P.keys, P.vals, P.keys2, P.vals2, P.p, P.key_bits, P.hist, P.hist_cap);
+  launch_bucket_ranges(s, P.keys, P.p, P.nbuckets, P.bstart);
+  launch_gather_obs(s, P.vals, P.p, P.k, P.yp, P.dd, P.o.var, P.o.a, P.o.b, P.o.c,
+                    P.prm->rweight, const_cast<double*>(ga.yp), const_cast<double*>(ga.d),
+                    const_cast<double*>(ga.rinv), const_cast<double*>(ga.oa),
+                    const_cast<double*>(ga.ob), const_cast<double*>(ga.oc));
+  if (P.prm->precision != MDA_PREC_FP64)
+    launch_mixed_obs(s, P.p, P.k, ga.yp, ga.oa, ga.ob, P.geo, const_cast<float*>(ga.y32),
+                     const_cast<double*>(ga.ux), const_cast<double*>(ga.uy),
+                     const_cast<double*>(ga.uz));
+}
+
+// Gram, in work-queue mode behind a descending-cost order of the columns.
+static void enqueue_gram(hipStream_t s, letkf_plan& P, int nblocks, bool queue) {
+  letkf_gram_args ga = P.ga;
+  ga.nblocks = nblocks;
+  if (queue) {
+    MDA_REQUIRE(P.queue, "work queue was not planned");
+    launch_column_cost(s, ga, P.ckeys, P.ccols);
+    radix_sort_pairs(s, P.ckeys, P.ccols, P.ckeys2, P.ccols2, P.ncol, 32, P.chist, P.chist_cap);
+    launch_zero(s, P.qcount, sizeof(uint32_t));
+    ga.order = P.ccols;  // the sort leaves the ordered pairs in the first arrays
+    ga.queue = P.qcount;
+  }
+  launch_letkf_gram(s, ga);
+}
+
+static void enqueue_solve_transform(hipStream_t s, letkf_plan& P, const hipEvent_t* ev_stage) {
+  letkf_solve_args sa = P.sa;
+  for (int i = 0; i < 3; ++i) sa.ev_stage[i] = ev_stage ? ev_stage[i] : nullptr;
+  launch_zero(s, sa.status, sizeof(int));
+  launch_zero(s, sa.sweeps, sizeof(int) * P.ncol);
+  launch_letkf_solve(s, sa);
+}
+
+static void enqueue_stats(hipStream_t s, letkf_plan& P) {
+  launch_zero(s, P.stats, sizeof(int64_t) * 9);
+  launch_reduce_int(s, P.ga.nloc, P.ncol, P.stats, nullptr);
+  launch_reduce_int(s, P.sweeps, P.ncol, P.stats + 3, nullptr);
+  if (P.eig_path) launch_reduce_int(s, P.sa.flag, P.ncol, P.stats + 6, nullptr);
+}
+
+static void finish_letkf(mda_ctx* c, letkf_plan& P, mda_letkf_diag* diag) {
+  int64_t hs[9] = {0};
+  int hstat = 0;
+  d2h(c, hs, P.stats, sizeof(hs));
+  d2h(c, &hstat, P.sa.status, sizeof(int));
+  c->last_T = P.sa.T; c->last_k = P.k; c->last_ncol = P.ncol; c->last_nloc = P.ga.nloc;
+  c->last_flag = P.eig_path ? P.sa.flag : nullptr;
+  if (hstat) throw mda_error(MDA_E_NUMERIC, "LETKF: a local Pa is not positive definite");
+  unstage_ensemble(c, P.ens, P.de);
+  if (diag) {
+    diag->n_columns = P.ncol;
+    diag->total_local_obs = hs[0];
+    diag->max_local_obs = hs[1];
+    diag->n_empty_columns = hs[2];
+    diag->max_jacobi_sweeps = (int32_t)hs[4];
+    diag->n_fallback_columns = P.eig_path ? P.ncol - hs[8] : 0;  // nonzero flags
+  }
 }
 
 int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens,
                       const mda_geometry* geom, const mda_obs* obs, const mda_hop* hop,
                       mda_letkf_diag* diag) {
   return guarded(c, [&] {
-    MDA_REQUIRE(prm && geom, "params and geometry are required");
-    MDA_REQUIRE(std::isfinite(prm->inflation) && prm->inflation > 0.0, "inflation must be > 0");
-    MDA_REQUIRE(std::isfinite(prm->radius) || prm->radius > 0.0, "radius must be > 0");
-    MDA_REQUIRE(prm->radius > 0.0, "radius must be > 0");
-    MDA_REQUIRE(prm->precision == MDA_PREC_FP64 || prm->precision == MDA_PREC_FP32 ||
-                    prm->precision == MDA_PREC_BF16X3,
-                "precision must be FP64, FP32 or BF16X3");
-    MDA_REQUIRE(geom->coord >= 0 && geom->coord <= 2, "unknown coordinate system");
     hipStream_t s = c->stream;
     MDA_HIP(hipEventRecord(c->ev[0], s));
-    dev_ensemble de = stage_ensemble(c, ens);
-    dev_obs o = stage_obs(c, obs, true);
-    dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
-    const int k = de.k;
-    const int64_t p = o.p, n = de.n, nx = ens->nx, ny = ens->ny, nlev = ens->nlev;
-    const int64_t ncol = nx * ny;
-    MDA_REQUIRE(k <= 64, "LETKF supports k <= 64 in this build");
-    const local_obs L = prepare_local(c, prm, de, ens, geom, o, h);
-    const double* ca = L.ca;
-    const double* cb = L.cb;
-    const double* cc = L.cc;
-    const double* rho_col = L.rho_col;
-    const bucket_grid bg = L.bg;
-    const int* bstart = L.bstart;
-    double *s_yp = L.s_yp, *s_d = L.s_d, *s_r = L.s_r, *s_a = L.s_a, *s_b = L.s_b, *s_c = L.s_c;
-    float* s_y32 = L.s_y32;
-    double *s_ux = L.s_ux, *s_uy = L.s_uy, *s_uz = L.s_uz;
-
-    // 3. per-column Gram
-    double* C = c->get<double>("col_C", (size_t)ncol * k * k);
-    double* bvec = c->get<double>("col_bvec", (size_t)ncol * k);
-    int* nloc = c->get<int>("col_nloc", ncol);
-    letkf_gram_args ga{};
-    ga.k = k; ga.ncol = ncol; ga.nx = nx; ga.coord = geom->coord;
-    ga.col_a = ca; ga.col_b = cb; ga.col_c = cc;
-    ga.grid = bg; ga.bstart = bstart;
-    ga.yp = s_yp; ga.d = s_d; ga.rinv = s_r; ga.oa = s_a; ga.ob = s_b; ga.oc = s_c;
-    ga.radius = prm->radius; ga.localization = prm->localization; ga.precision = prm->precision;
-    ga.y32 = s_y32; ga.ux = s_ux; ga.uy = s_uy; ga.uz = s_uz;
-    ga.C = C; ga.b = bvec; ga.nloc = nloc;
-    launch_letkf_gram(s, ga);
+    letkf_plan P = plan_letkf(c, prm, ens, geom, obs, hop, "", false);
+    MDA_HIP(hipEventRecord(c->ev[1], s));
+    enqueue_local(s, P);
+    MDA_HIP(hipEventRecord(c->ev[2], s));
+    enqueue_buckets(s, P);
+    MDA_HIP(hipEventRecord(c->ev[3], s));
+    enqueue_gram(s, P, 0, false);
     MDA_HIP(hipEventRecord(c->ev[4], s));
-
-    // 4. per-column solve
-    double* T = c->get<double>("col_T", (size_t)ncol * k * k);
-    int* sweeps = c->get<int>("col_sweeps", ncol);
-    int* status = c->get<int>("col_status", 1);
-    launch_zero(s, status, sizeof(int));
-    letkf_solve_args sa{};
-    sa.k = k; sa.ncol = ncol; sa.C = C; sa.b = bvec; sa.nloc = nloc;
-    sa.rho = prm->inflation; sa.rho_col = rho_col;
-    sa.inflation_form = prm->inflation_form; sa.sqrt_form = prm->sqrt_form;
-    sa.combine = prm->combine; sa.T = T; sa.sweeps = sweeps; sa.status = status;
-    sa.de = c->get<double>("eig_de", (size_t)ncol * 2 * k);
-    sa.lam = c->get<double>("eig_lam", (size_t)ncol * k);
-    sa.U = letkf_eig_supported(k) && prm->eig_method == MDA_EIG_AUTO &&
-                   prm->sqrt_form == MDA_SQRT_SYMMETRIC
-               ? c->get<double>("eig_U", (size_t)ncol * k * k)
-               : nullptr;
-    sa.flag = c->get<int>("eig_flag", ncol);
-    sa.method = prm->eig_method;
-    sa.ev_stage[0] = c->ev[7]; sa.ev_stage[1] = c->ev[8]; sa.ev_stage[2] = c->ev[9];
-    // eigenvalue noise of the Gram relative to ||A||: FP32 products with FP32
-    // accumulation, or BF16x3 products (2^-16 relative) with FP32 accumulation
-    sa.null_rel = prm->precision == MDA_PREC_FP32 ? 0x1p-22
-                  : prm->precision == MDA_PREC_BF16X3 ? 0x1p-17 : 0.0;
-    launch_zero(s, sweeps, sizeof(int) * ncol);
-    launch_letkf_solve(s, sa);
+    enqueue_solve_transform(s, P, &c->ev[7]);
     MDA_HIP(hipEventRecord(c->ev[5], s));
-
-    // 5. transform, in place
-    launch_letkf_transform(s, de.X, k, ncol, nlev, T);
+    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T);
     MDA_HIP(hipEventRecord(c->ev[6], s));
-
-    int64_t* stats = c->get<int64_t>("col_stats", 9);
-    launch_zero(s, stats, sizeof(int64_t) * 9);
-    launch_reduce_int(s, nloc, ncol, stats, nullptr);
-    launch_reduce_int(s, sweeps, ncol, stats + 3, nullptr);
-    const bool eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(k) &&
-                          prm->eig_method == MDA_EIG_AUTO;
-    if (eig_path) launch_reduce_int(s, sa.flag, ncol, stats + 6, nullptr);
-    int64_t hs[9] = {0};
-    int hstat = 0;
-    d2h(c, hs, stats, sizeof(hs));
-    d2h(c, &hstat, status, sizeof(int));
-    MDA_HIP(hipStreamSynchronize(s));
-    c->last_T = T; c->last_k = k; c->last_ncol = ncol; c->last_nloc = nloc;
-    c->last_flag = eig_path ? sa.flag : nullptr;
-    if (hstat) throw mda_error(MDA_E_NUMERIC, "LETKF: a local Pa is not positive definite");
-    unstage_ensemble(c, ens, de);
+    enqueue_stats(s, P);
+    finish_letkf(c, P, diag);
     if (diag) {
-      diag->n_columns = ncol;
-      diag->total_local_obs = hs[0];
-      diag->max_local_obs = hs[1];
-      diag->n_empty_columns = hs[2];
-      diag->max_jacobi_sweeps = (int32_t)hs[4];
-      diag->n_fallback_columns = eig_path ? ncol - hs[8] : 0;  // nonzero flags
       diag->ms_obs = elapsed(c->ev[1], c->ev[2]);
       diag->ms_bucket = elapsed(c->ev[2], c->ev[3]);
       diag->ms_gram = elapsed(c->ev[3], c->ev[4]);
       diag->ms_solve = elapsed(c->ev[4], c->ev[5]);
       diag->ms_transform = elapsed(c->ev[5], c->ev[6]);
       diag->ms_total = elapsed(c->ev[0], c->ev[6]);
-      if (eig_path) {
+      if (P.eig_path) {
         diag->ms_eig_tridiag = elapsed(c->ev[4], c->ev[7]);
         diag->ms_eig_values = elapsed(c->ev[7], c->ev[8]);
         diag->ms_eig_vectors = elapsed(c->ev[8], c->ev[9]);
@@ -627,6 +687,163 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
         diag->ms_eig_tridiag = diag->ms_eig_values = diag->ms_eig_vectors = 0.0;
         diag->ms_solve_jacobi = diag->ms_solve;
       }
+    }
+  });
+}
+
+// Batched LETKF: independent analyses (any mix of k, precision and
+// workload) run as concurrent branches of one hipGraph, the Gram of each in
+// work-queue mode behind a descending-cost column order, with the Gram
+// launch shape chosen by timing candidates on the first call of a workload.
+static std::string tune_key(const letkf_plan& P) {
+  return "k" + std::to_string(P.k) + ":prec" + std::to_string(P.prm->precision) + ":ncol" +
+         std::to_string(P.ncol) + ":p" + std::to_string(P.p);
+}
+
+static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_queue) {
+  hipStream_t s = c->stream;
+  enqueue_local(s, P);
+  enqueue_buckets(s, P);
+  const int cand_blocks[] = {0, 512, 1024, 2048};
+  std::pair<int, int> best{0, 0};
+  float best_ms = 1e30f;
+  for (int q = 0; q <= (allow_queue ? 1 : 0); ++q) {
+    for (int nb : cand_blocks) {
+      if (q == 0 && nb != 0) continue;  // grid-stride mode: one wave per column
+      enqueue_gram(s, P, nb, q != 0);   // warm
+      MDA_HIP(hipEventRecord(c->ev[10], s));
+      enqueue_gram(s, P, nb, q != 0);
+      MDA_HIP(hipEventRecord(c->ev[11], s));
+      MDA_HIP(hipEventSynchronize(c->ev[11]));
+      const float ms = elapsed(c->ev[10], c->ev[11]);
+      if (ms < best_ms) { best_ms = ms; best = {nb, q}; }
+    }
+  }
+  return best;
+}
+
+static void enqueue_plan(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg) {
+  enqueue_local(s, P);
+  enqueue_buckets(s, P);
+  enqueue_gram(s, P, cfg.first, cfg.second != 0);
+  enqueue_solve_transform(s, P, nullptr);
+  launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T);
+  enqueue_stats(s, P);
+}
+
+int mda_letkf_analyse_batch(mda_ctx* c, int32_t nb, const mda_letkf_params* prm,
+                            mda_ensemble* ens, const mda_geometry* geom, const mda_obs* obs,
+                            const mda_hop* hop, mda_letkf_diag* diag, int32_t flags,
+                            mda_batch_info* info) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(nb >= 1 && nb <= 16, "batch size must be 1..16");
+    MDA_REQUIRE(prm && ens && geom && obs && hop, "null batch arrays");
+    MDA_REQUIRE((flags & ~(MDA_BATCH_GRAPH | MDA_BATCH_QUEUE | MDA_BATCH_AUTOTUNE)) == 0,
+                "unknown batch flags");
+    for (int i = 0; i < nb; ++i)
+      MDA_REQUIRE(ens[i].on_device && obs[i].on_device && hop[i].on_device,
+                  "batched analyses take device-resident ensembles, observations and H");
+    const bool use_graph = flags & MDA_BATCH_GRAPH, use_queue = flags & MDA_BATCH_QUEUE;
+    hipStream_t s = c->stream;
+    std::vector<letkf_plan> plans;
+    plans.reserve(nb);
+    for (int i = 0; i < nb; ++i)
+      plans.push_back(plan_letkf(c, &prm[i], &ens[i], &geom[i], &obs[i], &hop[i],
+                                 "b" + std::to_string(i) + ":", use_queue));
+    std::vector<std::pair<int, int>> cfg(nb, {0, use_queue ? 1 : 0});
+    for (int i = 0; i < nb; ++i) {
+      const std::string key = tune_key(plans[i]) + (use_queue ? ":q" : "");
+      auto it = c->tune.find(key);
+      if (it != c->tune.end()) {
+        cfg[i] = it->second;
+      } else if (flags & MDA_BATCH_AUTOTUNE) {
+        cfg[i] = c->tune[key] = autotune_gram(c, plans[i], use_queue);
+      }
+    }
+    // streams and fork/join events for concurrent branches
+    while ((int)c->side.size() < nb - 1) {
+      hipStream_t st = nullptr;
+      MDA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      c->side.push_back(st);
+    }
+    while ((int)c->fj.size() < nb + 1) {
+      hipEvent_t e = nullptr;
+      MDA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      c->fj.push_back(e);
+    }
+    auto enqueue_all = [&] {
+      MDA_HIP(hipEventRecord(c->fj[0], s));
+      for (int i = 1; i < nb; ++i) MDA_HIP(hipStreamWaitEvent(c->side[i - 1], c->fj[0], 0));
+      for (int i = 0; i < nb; ++i) enqueue_plan(i == 0 ? s : c->side[i - 1], plans[i], cfg[i]);
+      for (int i = 1; i < nb; ++i) {
+        MDA_HIP(hipEventRecord(c->fj[i], c->side[i - 1]));
+        MDA_HIP(hipStreamWaitEvent(s, c->fj[i], 0));
+      }
+    };
+    int replayed = 0;
+    MDA_HIP(hipStreamSynchronize(s));
+    if (use_graph) {
+      std::string sig = std::to_string(nb);
+      for (int i = 0; i < nb; ++i) {
+        const letkf_plan& P = plans[i];
+        char buf[512];
+        snprintf(buf, sizeof(buf), "|%p,%p,%p,%p,%p,%p,%d,%lld,%lld,%lld,%d,%d,%d,%d,%d,%d,%.17g,%.17g,%p,%d,%d",
+                 (const void*)P.de.X, (const void*)P.o.yo, (const void*)P.ga.C, (const void*)P.sa.T,
+                 (const void*)P.h.row_ptr, (const void*)P.h.hx, P.k, (long long)P.p,
+                 (long long)P.ncol, (long long)P.nlev, P.prm->precision, P.prm->localization,
+                 P.prm->rweight, P.prm->inflation_form, P.prm->sqrt_form, P.prm->combine,
+                 P.prm->inflation, P.prm->radius, (const void*)P.sa.rho_col, cfg[i].first,
+                 cfg[i].second);
+        sig += buf;
+      }
+      auto it = c->graphs.find(sig);
+      if (it == c->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        MDA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_all();
+        } catch (...) {
+          hipGraph_t g2 = nullptr;
+          (void)hipStreamEndCapture(s, &g2);
+          if (g2) (void)hipGraphDestroy(g2);
+          throw;
+        }
+        MDA_HIP(hipStreamEndCapture(s, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MDA_HIP(e);
+        it = c->graphs.emplace(sig, exec).first;
+      } else {
+        replayed = 1;
+      }
+      MDA_HIP(hipEventRecord(c->ev[0], s));
+      MDA_HIP(hipGraphLaunch(it->second, s));
+      MDA_HIP(hipEventRecord(c->ev[6], s));
+    } else {
+      MDA_HIP(hipEventRecord(c->ev[0], s));
+      enqueue_all();
+      MDA_HIP(hipEventRecord(c->ev[6], s));
+    }
+    MDA_HIP(hipStreamSynchronize(s));
+    const double ms = elapsed(c->ev[0], c->ev[6]);
+    for (int i = 0; i < nb; ++i) {
+      mda_letkf_diag* d = diag ? &diag[i] : nullptr;
+      finish_letkf(c, plans[i], d);
+      if (d) {
+        d->ms_total = ms;
+        d->ms_obs = d->ms_bucket = d->ms_gram = d->ms_solve = d->ms_transform = 0.0;
+        d->ms_eig_tridiag = d->ms_eig_values = d->ms_eig_vectors = d->ms_solve_jacobi = 0.0;
+      }
+      if (info && i < 16) {
+        info->gram_blocks[i] = cfg[i].first;
+        info->gram_queue[i] = cfg[i].second;
+      }
+    }
+    if (info) {
+      info->ms_total = ms;
+      info->graph_replayed = replayed;
+      info->n_graphs = (int32_t)c->graphs.size();
     }
   });
 }
@@ -722,19 +939,16 @@ int mda_letkf_inflation_features(mda_ctx* c, const mda_letkf_params* prm,
     mda_letkf_params fp = *prm;
     fp.precision = MDA_PREC_FP64;
     fp.rho_column = nullptr;
-    dev_ensemble de = stage_ensemble(c, ens);
-    dev_obs o = stage_obs(c, obs, true);
-    dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
-    const int64_t ncol = ens->nx * ens->ny;
-    const local_obs L = prepare_local(c, &fp, de, ens, geom, o, h);
-    letkf_gram_args ga{};
-    ga.k = de.k; ga.ncol = ncol; ga.nx = ens->nx; ga.coord = geom->coord;
-    ga.col_a = L.ca; ga.col_b = L.cb; ga.col_c = L.cc;
-    ga.grid = L.bg; ga.bstart = L.bstart;
-    ga.yp = L.s_yp; ga.d = L.s_d; ga.rinv = L.s_r; ga.oa = L.s_a; ga.ob = L.s_b; ga.oc = L.s_c;
-    ga.radius = prm->radius; ga.localization = prm->localization;
+    letkf_plan P = plan_letkf(c, &fp, const_cast<mda_ensemble*>(ens), geom, obs, hop, "",
+                              false);
+    enqueue_local(s, P);
+    enqueue_buckets(s, P);
+    const int64_t ncol = P.ncol;
+    const dev_obs& o = P.o;
+    const dev_ensemble& de = P.de;
+    const letkf_gram_args& ga = P.ga;
     double* sp = c->get<double>("feat_sp", o.p);
-    launch_obs_spread(s, L.s_yp, o.p, de.k, sp);
+    launch_obs_spread(s, ga.yp, o.p, de.k, sp);
     const double* rp = to_device(c, "feat_rho_prev", rho_prev, ncol, prm->rho_on_device != 0);
     float* f = features_on_device ? features : c->get<float>("feat_out", (size_t)ncol * 8);
     launch_letkf_features(s, ga, sp, de.X, ens->nlev, rp, f);

@@ -289,6 +289,43 @@ __device__ inline col_window column_window(const letkf_gram_args& a, int64_t g) 
   return w;
 }
 
+// Column iteration of the Gram kernels: a grid-stride walk, or (work-queue
+// mode) lane 0 of each wave takes the next position of a cost-sorted order
+// with a global atomic counter; every wave exits once the counter passes ncol.
+__device__ inline int64_t queue_next(uint32_t* q) {
+  uint32_t v = 0;
+  if ((threadIdx.x & 63) == 0) v = atomicAdd(q, 1u);
+  return (int64_t)__shfl(v, 0);
+}
+
+// Cost of a column = observations in the buckets of its candidate window
+// (an upper bound of its local count, proportional to its Gram work); the
+// key sorts columns by descending cost.
+__global__ void k_column_cost(letkf_gram_args a, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ cols) {
+  const bucket_grid& G = a.grid;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.ncol;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const col_window cw = column_window(a, g);
+    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
+    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
+    int64_t cost = 0;
+    for (int row = cw.row0; row <= cw.row1; ++row)
+      for (int sg = 0; sg < cw.sl.n; ++sg)
+        cost += a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1] -
+                a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
+    keys[g] = 0xffffffffu - (uint32_t)(cost < 0xffffffffLL ? cost : 0xffffffffLL);
+    cols[g] = (uint32_t)g;
+  }
+}
+
+void launch_column_cost(hipStream_t s, const letkf_gram_args& a, uint32_t* keys,
+                        uint32_t* cols) {
+  hipLaunchKernelGGL(k_column_cost, dim3(grid_for(a.ncol, 256, 8192)), dim3(256), 0, s, a, keys,
+                     cols);
+  MDA_LAUNCH_CHECK();
+}
+
 template <int KT>
 __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   constexpr int NT = KT * (KT + 1) / 2;
@@ -299,7 +336,9 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   const int k = a.k;
   const bucket_grid& G = a.grid;
   const double cgc = a.radius / 2.0;
-  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
+  int64_t it = a.queue ? queue_next(a.queue) : (int64_t)blockIdx.x * 4 + wv;
+  for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + (int64_t)gridDim.x * 4) {
+    const int64_t g = a.order ? (int64_t)a.order[it] : it;
     d4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
@@ -460,7 +499,9 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
   unsigned char* stg = stg_raw[wv];
-  for (int64_t g = vb * 4 + wv; g < a.ncol; g += nb * 4) {
+  int64_t it = a.queue ? queue_next(a.queue) : vb * 4 + wv;
+  for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + nb * 4) {
+    const int64_t g = a.order ? (int64_t)a.order[it] : it;
     f4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
@@ -863,6 +904,8 @@ void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, doub
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   unsigned nb = (unsigned)((a.ncol + 3) / 4);
   if (nb > 262144) nb = 262144;
+  if (a.nblocks > 0 && (unsigned)a.nblocks < nb) nb = (unsigned)a.nblocks;
+  MDA_REQUIRE(!a.queue || a.ncol < ((int64_t)1 << 32) - 262144 * 4, "too many columns for the queue");
   int kt = (a.k + 15) / 16;
   if (a.precision != MDA_PREC_FP64) {
     MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "mixed Gram inputs missing");

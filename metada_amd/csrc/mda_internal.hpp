@@ -48,6 +48,13 @@ struct mda_ctx {
   };
   std::map<std::string, buf> ws;  // named device workspace, grown on demand
   hipEvent_t ev[12] = {};
+  // batched analyses: extra streams (default priority) for concurrent
+  // analyses, fork/join events, instantiated graphs by signature, and the
+  // autotuned Gram launch (blocks, work queue) by workload key
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> fj;
+  std::map<std::string, hipGraphExec_t> graphs;
+  std::map<std::string, std::pair<int, int>> tune;
   // state of the last LETKF call (for mda_letkf_last_transforms)
   int last_k = 0;
   int64_t last_ncol = 0;
@@ -210,8 +217,17 @@ struct letkf_gram_args {
   double* C;    // out [ncol][k][k]
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
+  // work-queue mode: persistent waves take columns order[0], order[1], ...
+  // through the counter `queue` (zeroed before the launch)
+  const uint32_t* order;
+  uint32_t* queue;
+  int nblocks;  // launch size (0: one wave per column)
 };
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
+// per-column cost estimate (observations in the candidate buckets) as
+// descending-order sort keys, and column ids
+void launch_column_cost(hipStream_t s, const letkf_gram_args& a, uint32_t* keys,
+                        uint32_t* cols);
 // learned-inflation inputs: per-observation background variance of the
 // sorted Y', then the 8 local statistics of every column
 void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, double* sp);
