@@ -6,7 +6,7 @@
 //       the Householder vectors u_j and tau_j go to the reflector buffer U.
 //   E2  k_trd_eigvals: eigenvalues of T by implicit QL with Wilkinson shifts,
 //       one LANE per column so the serial scalar recurrence of 64 columns runs
-//       side by side.
+//       side by side, d and e in registers.
 //   E3  k_trd_vectors: eigenvector s_j of T for eigenvalue lambda_j by a
 //       twisted factorisation of T - lambda_j I (one lane per eigenvalue),
 //       residual and near-pair orthogonality checks, back-transformation
@@ -159,76 +159,145 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
 // ---------------------------------------------------------------------------
 // E2: eigenvalues of the symmetric tridiagonal (d, e) by implicit QL with a
 // Wilkinson-type shift (Golub & Van Loan, sec. 8.3, QL orientation), one lane
-// per column, arrays in LDS as [index][lane].  Ascending output.
+// per column.  d and e live in registers; every lane runs the same static
+// index sequence and applies only the steps inside its own unreduced block
+// [l, m], so the arrays are addressed by compile-time indices (no LDS, one
+// wavefront per SIMD).  Steps outside every lane's block are skipped by a
+// wave-uniform branch.  Ascending output by an odd-even merge network.
 // ---------------------------------------------------------------------------
+
+// sqrt(f^2 + g^2) without the library hypot; rescaled only when the squares
+// would leave the normal range.
+__device__ inline double hyp2(double f, double g) {
+  const double a = fabs(f), b = fabs(g);
+  const double t = fmax(a, b);
+  double r = sqrt(fma(f, f, g * g));
+  if (t > 1e150 || (t < 1e-150 && t > 0.0)) {
+    const double u = fmin(a, b) / t;
+    r = t * sqrt(fma(u, u, 1.0));
+  }
+  return r;
+}
+
+__device__ inline bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
+
 template <int N>
-__global__ __launch_bounds__(64) void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc,
-                                                    const double* __restrict__ de,
-                                                    double* __restrict__ lam,
-                                                    int* __restrict__ flag) {
+__device__ inline void cmp_swap(double (&d)[N], int i, int j) {
+  const double a = d[i], b = d[j];
+  d[i] = fmin(a, b);
+  d[j] = fmax(a, b);
+}
+
+// Batcher's odd-even merge sort over the next power of two, comparators that
+// touch an index >= N dropped (equivalent to padding with +inf).
+template <int N>
+__device__ inline void sort_network(double (&d)[N]) {
+  constexpr int P = N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+#pragma unroll
+  for (int p = 1; p < P; p <<= 1) {
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % p; j + k < P; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          const int a = i + j, b = i + j + k;
+          if (b < N && a / (2 * p) == b / (2 * p)) cmp_swap(d, a, b);
+        }
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1)))
+void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+                   double* __restrict__ lam, int* __restrict__ flag) {
   constexpr int n = N;
-  __shared__ double sd[N * 64], se[N * 64];
   const int lane = threadIdx.x;
   const int64_t g = (int64_t)blockIdx.x * 64 + lane;
   const bool active = g < ncol && nloc[g] != 0;
-  double* d = sd + lane;  // d[i * 64]
-  double* e = se + lane;
-  if (active) {
-    const double* dg = de + (size_t)g * 2 * n;
-    for (int i = 0; i < n; ++i) { d[i * 64] = dg[i]; e[i * 64] = dg[n + i]; }
-    e[(n - 1) * 64] = 0.0;
-    bool fail = false;
-    const double eps = 2.220446049250313e-16;
-    for (int l = 0; l < n && !fail; ++l) {
-      int iter = 0;
-      while (true) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          double dd = fabs(d[m * 64]) + fabs(d[(m + 1) * 64]);
-          if (fabs(e[m * 64]) <= eps * dd) break;
+  const double* dg = de + (size_t)(active ? g : 0) * 2 * n;
+  double d[N], e[N];
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    d[i] = active ? dg[i] : 0.0;
+    e[i] = (active && i < n - 1) ? dg[n + i] : 0.0;
+  }
+  const double eps = 2.220446049250313e-16;
+  // inactive lanes hold zeros: every e is negligible and l runs to n-1 at once
+  int l = 0, iter = 0;
+  bool fail = false, done = false;
+  while (true) {
+    // negligible off-diagonals e_i (bit i), restricted to i >= the smallest l
+    uint64_t small = 0;
+#pragma unroll
+    for (int i = 0; i < n - 1; ++i) {
+      if (!wave_any(!done && l <= i)) continue;
+      const bool z = fabs(e[i]) <= eps * (fabs(d[i]) + fabs(d[i + 1]));
+      small |= z ? (1ull << i) : 0ull;
+    }
+    // leading converged entries: d_l is an eigenvalue while e_l is negligible
+    if (!done) {
+      const int lp = __builtin_ctzll(~small & (~0ull << l));  // bits >= n-1 of ~small are set
+      if (lp != l) { l = lp; iter = 0; }
+      if (l >= n - 1) done = true;
+    }
+    // end of the unreduced block starting at l
+    const uint64_t ms = small & (done ? 0ull : (~0ull << l));
+    const int m = ms ? __builtin_ctzll(ms) : n - 1;
+    if (!done && ++iter > 60) { fail = true; done = true; }
+    if (!wave_any(!done)) break;
+    double dl = 0.0, dl1 = 0.0, el = 1.0, dm = 0.0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      if (!wave_any(!done && l <= i && i <= m)) continue;
+      if (i == l) {
+        dl = d[i];
+        el = e[i];
+        if (i + 1 < n) dl1 = d[i + 1];
+      }
+      if (i == m) dm = d[i];
+    }
+    // shift from the leading 2x2 of the block
+    double gg = (dl1 - dl) / (2.0 * el);
+    double rr = hyp2(gg, 1.0);
+    gg = dm - dl + el / (gg + (gg >= 0.0 ? rr : -rr));
+    double s = 1.0, c = 1.0, p = 0.0;
+    bool run = !done;  // cleared when this lane's sweep ends early (underflow)
+#pragma unroll
+    for (int i = n - 2; i >= 0; --i) {
+      const bool act = run && l <= i && i < m;
+      if (!wave_any(act)) continue;
+      const double f = s * e[i], b = c * e[i];
+      const double r = hyp2(f, gg);
+      const bool under = act && r == 0.0;     // split and restart the block
+      const bool rot = act && r != 0.0;
+      if (act) e[i + 1] = (i == m - 1 || under) ? 0.0 : r;  // e_m ends at zero
+      if (under) { d[i + 1] -= p; run = false; }
+      const double inv = 1.0 / r;
+      const double sn = f * inv, cn = gg * inv;
+      const double g2 = d[i + 1] - p;
+      const double r2 = (d[i] - g2) * sn + 2.0 * cn * b;
+      const double pn = sn * r2;
+      if (rot) {
+        d[i + 1] = g2 + pn;
+        gg = cn * r2 - b;
+        s = sn;
+        c = cn;
+        p = pn;
+        if (i == l) {  // end of the sweep
+          d[i] -= p;
+          e[i] = gg;
         }
-        if (m == l) break;
-        if (++iter > 60) { fail = true; break; }
-        // shift from the leading 2x2 of the unreduced block
-        double gg = (d[(l + 1) * 64] - d[l * 64]) / (2.0 * e[l * 64]);
-        double rr = hypot(gg, 1.0);
-        gg = d[m * 64] - d[l * 64] + e[l * 64] / (gg + (gg >= 0.0 ? rr : -rr));
-        double s = 1.0, c = 1.0, p = 0.0;
-        int i = m - 1;
-        bool early = false;
-        for (; i >= l; --i) {
-          double f = s * e[i * 64], b = c * e[i * 64];
-          rr = hypot(f, gg);
-          e[(i + 1) * 64] = rr;
-          if (rr == 0.0) {  // underflow: split and restart
-            d[(i + 1) * 64] -= p;
-            e[m * 64] = 0.0;
-            early = true;
-            break;
-          }
-          s = f / rr;
-          c = gg / rr;
-          gg = d[(i + 1) * 64] - p;
-          rr = (d[i * 64] - gg) * s + 2.0 * c * b;
-          p = s * rr;
-          d[(i + 1) * 64] = gg + p;
-          gg = c * rr - b;
-        }
-        if (early) continue;
-        d[l * 64] -= p;
-        e[l * 64] = gg;
-        e[m * 64] = 0.0;
       }
     }
-    // ascending insertion sort
-    for (int a = 1; a < n; ++a) {
-      double v = d[a * 64];
-      int b = a - 1;
-      while (b >= 0 && d[b * 64] > v) { d[(b + 1) * 64] = d[b * 64]; --b; }
-      d[(b + 1) * 64] = v;
-    }
+  }
+  sort_network<N>(d);
+  if (active) {
     double* lg = lam + (size_t)g * n;
-    for (int i = 0; i < n; ++i) lg[i] = d[i * 64];
+#pragma unroll
+    for (int i = 0; i < n; ++i) lg[i] = d[i];
     if (fail) flag[g] = 1;  // QL did not converge
   }
 }
