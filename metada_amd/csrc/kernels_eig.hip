@@ -51,6 +51,32 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// One DPP move of a double (both halves); lanes whose source is out of range
+// or whose row is masked off read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned wave-uniform: prefix sums within rows of
+// 16 (row_shr 1, 2, 4, 8), row 15 -> rows 1, 3 (row_bcast:15), lane 31 ->
+// rows 2, 3 (row_bcast:31); lane 63 then holds the total.
+__device__ inline double wave_total(double v) {
+  v += dpp_f64<0x111>(v);
+  v += dpp_f64<0x112>(v);
+  v += dpp_f64<0x114>(v);
+  v += dpp_f64<0x118>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // ---------------------------------------------------------------------------
 // E1: Householder tridiagonalisation.  For j = 0..n-3 with m = n-j-1:
 //   x = A[j+1:, j],  alpha = -sign(x_0) ||x||,  u = x - alpha e_1,
@@ -100,17 +126,15 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
         }
         wave_lds_sync();
         const double xi = (lane > j && lane < n) ? row[lane] : 0.0;  // A[lane][j] = A[j][lane]
-        const double nrm2 = wave_sum(xi * xi);
+        const double nrm2 = wave_total(xi * xi);
         const double x0 = row[j + 1];
-        double alpha = 0.0, tau = 0.0, ui = 0.0;
-        if (nrm2 > 0.0) {
-          alpha = (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2);
-          ui = xi - (lane == j + 1 ? alpha : 0.0);
-          tau = 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha);
-        }
+        const bool nz = nrm2 > 0.0;
+        const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
+        const double ui = nz ? xi - (lane == j + 1 ? alpha : 0.0) : 0.0;
+        const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
         if (lane == 0) { dg[j] = row[j]; dg[n + j] = alpha; }
         if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
-        if (tau == 0.0) { wave_lds_sync(); continue; }  // x = 0: H = I
+        // x = 0: tau = u = 0, the step below leaves A unchanged (H = I)
         ub[lane] = ui;
         wave_lds_sync();
         double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
@@ -122,7 +146,7 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
           if (c + 3 < N) p3 += ar[c + 3 < N ? c + 3 : c] * ub[c + 3 < N ? c + 3 : c];
         }
         const double p = tau * ((p0 + p1) + (p2 + p3));
-        const double K = 0.5 * tau * wave_sum(p * ui);
+        const double K = 0.5 * tau * wave_total(p * ui);
         const double wi = (lane > j && lane < n) ? p - K * ui : 0.0;
         wb[lane] = wi;
         wave_lds_sync();
