@@ -46,11 +46,6 @@ __device__ inline void l2_prefetch(const void* src, int64_t bytes, unsigned* sin
                                      (__attribute__((address_space(3))) void*)sink, 4, 0, 0);
 }
 
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
 // One DPP move of a double (both halves); lanes whose source is out of range
 // or whose row is masked off read 0.
 template <int CTRL, int ROW_MASK = 0xf>
@@ -58,6 +53,13 @@ __device__ inline double dpp_f64(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
@@ -71,10 +73,7 @@ __device__ inline double wave_total(double v) {
   v += dpp_f64<0x118>(v);
   v += dpp_f64<0x142, 0xa>(v);
   v += dpp_f64<0x143, 0xc>(v);
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, 63);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  return readlane_f64(v, 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -82,10 +81,10 @@ __device__ inline double wave_total(double v) {
 //   x = A[j+1:, j],  alpha = -sign(x_0) ||x||,  u = x - alpha e_1,
 //   tau = 2 / u^T u,  p = tau A' u,  w = p - (tau/2)(p^T u) u,
 //   A' <- A' - u w^T - w u^T   (A' = A[j+1:, j+1:]),  e_j = alpha.
-// Lane r keeps row r of A in registers (static column index); column j is
-// read as row j (A is symmetric) through one LDS row.  The reflector j is
-// stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j, and tau_j in
-// U[j][0] (never a reflector position).
+// Lane r keeps row r of A in registers (static column index); lane r's
+// entry of column j is picked from its registers in the step before.  The
+// reflector j is stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j,
+// and tau_j in U[j][0] (never a reflector position).
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
@@ -110,6 +109,10 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
     }
     double* U = Uout + (size_t)g * n * n;
     double* dg = de + (size_t)g * 2 * n;
+    // xc: column j of the current A, lane r holding A[r][j]; taken from the
+    // registers during the previous step's update (static index, selected
+    // on the uniform j), so no row travels through LDS.
+    double xc = ar[0];
     // Steps j in chunks of 16: within chunk ch every quantity indexed by a
     // column c <= j is zero or unused, so the static column ranges start at
     // 16 ch (fewer row writes, dot-product and update terms).
@@ -120,19 +123,14 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
       const int jend = (16 * ch + 16 < n - 2) ? 16 * ch + 16 : n - 2;
 #pragma unroll 1
       for (int j = c0; j < jend; ++j) {
-        if (lane == j) {
-#pragma unroll
-          for (int c = c0; c < N; ++c) row[c] = ar[c];
-        }
-        wave_lds_sync();
-        const double xi = (lane > j && lane < n) ? row[lane] : 0.0;  // A[lane][j] = A[j][lane]
+        const double xi = (lane > j && lane < n) ? xc : 0.0;
         const double nrm2 = wave_total(xi * xi);
-        const double x0 = row[j + 1];
+        const double x0 = readlane_f64(xc, j + 1);
         const bool nz = nrm2 > 0.0;
         const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
         const double ui = nz ? xi - (lane == j + 1 ? alpha : 0.0) : 0.0;
         const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
-        if (lane == 0) { dg[j] = row[j]; dg[n + j] = alpha; }
+        if (lane == 0) { dg[j] = readlane_f64(xc, j); dg[n + j] = alpha; }
         if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
         // x = 0: tau = u = 0, the step below leaves A unchanged (H = I)
         ub[lane] = ui;
@@ -154,6 +152,7 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
           ar[c] -= ui * wb[c] + wi * ub[c];
+          if (c > c0 && c <= c0 + 16 && c == j + 1) xc = ar[c];
         }
         wave_lds_sync();
       }
