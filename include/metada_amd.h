@@ -236,7 +236,10 @@ int mda_letkf_analyse_batch(mda_ctx* ctx, int32_t nb, const mda_letkf_params* pr
 
 /* Per-column k x k transforms T (X_a = xbar + X' T) of the last LETKF call,
  * for the listed columns, row-major [ncols][k][k] (host).  Diagnostics and
- * parity tests; valid until the next call on the context. */
+ * parity tests; valid until the next call on the context.  Not kept when the
+ * analysis ran its columns in passes (k > 64 on grids over 65,536 columns,
+ * or a pass size set by the environment variable MDA_LETKF_CHUNK_COLS):
+ * MDA_E_ARG then. */
 int mda_letkf_last_transforms(mda_ctx* ctx, const int64_t* cols, int64_t ncols, double* T_out);
 
 /* Number of local observations (within the localisation radius) of every

@@ -2,6 +2,7 @@
 // the context's device workspace when needed, enqueues the kernels on the
 // context's stream and converts every error into an MDA_E_* code.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <limits>
 
@@ -192,6 +193,8 @@ int mda_ctx_destroy(mda_ctx* c) {
     (void)hipStreamDestroy(st);
   }
   for (auto e : c->fj)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : c->evpool)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipSetDevice(prev);
@@ -426,6 +429,9 @@ struct letkf_plan {
   dev_hop h;
   int k = 0;
   int64_t p = 0, n = 0, nx = 0, ny = 0, nlev = 0, ncol = 0;
+  // columns per pass of Gram -> solve -> transform: the k x k per-column
+  // buffers (C, U, T) hold one pass; ncol when the whole grid fits
+  int64_t chunk = 0;
   const double *rcoord = nullptr, *ccoord = nullptr;
   bucket_grid bg{};
   int nbuckets = 0, key_bits = 1;
@@ -469,7 +475,9 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   const int64_t p = P.p = P.o.p;
   P.n = P.de.n; P.nx = ens->nx; P.ny = ens->ny; P.nlev = ens->nlev;
   const int64_t ncol = P.ncol = P.nx * P.ny;
-  MDA_REQUIRE(k <= 64, "LETKF supports k <= 64 in this build");
+  MDA_REQUIRE(k <= 128, "LETKF supports k <= 128 in this build");
+  MDA_REQUIRE(k <= 64 || prm->precision == MDA_PREC_FP64,
+              "FP32 / BF16x3 contractions support k <= 64 in this build");
   const bool gd = geom->on_device != 0;
   const double* ca = to_device(c, pre + "geom_a", geom->col_a, ncol, gd);
   const double* cb = to_device(c, pre + "geom_b", geom->col_b, ncol, gd);
@@ -542,7 +550,16 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
       ga.uz = c->get<double>(pre + "s_uz", p);
     }
   }
-  ga.C = c->get<double>(pre + "col_C", (size_t)ncol * k * k);
+  // Column passes: at k > 64 the per-column k x k buffers of the whole grid
+  // would not fit in HBM next to the ensemble (3 x 69 GB at k = 128 and
+  // 524,288 columns), so the columns run in passes of 65,536.
+  P.chunk = k > 64 ? std::min<int64_t>(ncol, 65536) : ncol;
+  if (const char* ev = std::getenv("MDA_LETKF_CHUNK_COLS")) {
+    const long long v = std::atoll(ev);
+    if (v > 0) P.chunk = std::min<int64_t>(ncol, v);
+  }
+  const int64_t nck = P.chunk;
+  ga.C = c->get<double>(pre + "col_C", (size_t)nck * k * k);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
   P.queue = queue;
@@ -561,15 +578,25 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   sa.rho = prm->inflation; sa.rho_col = rho_col;
   sa.inflation_form = prm->inflation_form; sa.sqrt_form = prm->sqrt_form;
   sa.combine = prm->combine;
-  sa.T = c->get<double>(pre + "col_T", (size_t)ncol * k * k);
+  sa.T = c->get<double>(pre + "col_T", (size_t)nck * k * k);
   P.sweeps = sa.sweeps = c->get<int>(pre + "col_sweeps", ncol);
   sa.status = c->get<int>(pre + "col_status", 1);
   sa.de = c->get<double>(pre + "eig_de", (size_t)ncol * 2 * k);
   sa.lam = c->get<double>(pre + "eig_lam", (size_t)ncol * k);
   P.eig_path = prm->sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(k) &&
                prm->eig_method == MDA_EIG_AUTO;
-  sa.U = P.eig_path ? c->get<double>(pre + "eig_U", (size_t)ncol * k * k) : nullptr;
+  sa.U = P.eig_path ? c->get<double>(pre + "eig_U", (size_t)nck * k * k) : nullptr;
   sa.flag = c->get<int>(pre + "eig_flag", ncol);
+  sa.Zs = P.eig_path && k > 64 ? c->get<double>(pre + "eig_Z", (size_t)nck * k * k) : nullptr;
+  // Jacobi (fallback, or the Jacobi-only method) beyond the LDS capacity runs
+  // in a global per-block scratch
+  {
+    const int64_t kp = (k + 1) & ~1;
+    if (sizeof(double) * 2 * kp * (kp + 1) > 96 * 1024) {
+      sa.jblocks = 1024;
+      sa.jscratch = c->get<double>(pre + "jac_scratch", (size_t)sa.jblocks * 2 * kp * (kp + 1));
+    }
+  }
   sa.method = prm->eig_method;
   // eigenvalue noise of the Gram relative to ||A||: FP32 products with FP32
   // accumulation, or BF16x3 products (2^-16 relative) with FP32 accumulation
@@ -602,14 +629,44 @@ static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
                      const_cast<double*>(ga.uz));
 }
 
-// Gram, in work-queue mode behind a descending-cost order of the columns.
-static void enqueue_gram(hipStream_t s, letkf_plan& P, int nblocks, bool queue) {
+// The plan's per-column arguments restricted to the pass of columns
+// [c0, c0 + nc): pointers into the whole-grid arrays move to column c0, the
+// k x k buffers (C, U, T) are the pass's own.
+static letkf_gram_args gram_pass(const letkf_plan& P, int64_t c0, int64_t nc) {
   letkf_gram_args ga = P.ga;
+  ga.col0 = c0;
+  ga.ncol = nc;
+  if (ga.col_a) ga.col_a += c0;
+  if (ga.col_b) ga.col_b += c0;
+  if (ga.col_c) ga.col_c += c0;
+  ga.b += c0 * P.k;
+  ga.nloc += c0;
+  return ga;
+}
+
+static letkf_solve_args solve_pass(const letkf_plan& P, int64_t c0, int64_t nc) {
+  letkf_solve_args sa = P.sa;
+  sa.ncol = nc;
+  sa.b += c0 * P.k;
+  sa.nloc += c0;
+  if (sa.rho_col) sa.rho_col += c0;
+  sa.sweeps += c0;
+  sa.de += c0 * 2 * P.k;
+  sa.lam += c0 * P.k;
+  sa.flag += c0;
+  return sa;
+}
+
+// Gram of one pass, in work-queue mode behind a descending-cost order of
+// its columns.
+static void enqueue_gram(hipStream_t s, letkf_plan& P, int64_t c0, int64_t nc, int nblocks,
+                         bool queue) {
+  letkf_gram_args ga = gram_pass(P, c0, nc);
   ga.nblocks = nblocks;
   if (queue) {
     MDA_REQUIRE(P.queue, "work queue was not planned");
     launch_column_cost(s, ga, P.ckeys, P.ccols);
-    radix_sort_pairs(s, P.ckeys, P.ccols, P.ckeys2, P.ccols2, P.ncol, 32, P.chist, P.chist_cap);
+    radix_sort_pairs(s, P.ckeys, P.ccols, P.ckeys2, P.ccols2, nc, 32, P.chist, P.chist_cap);
     launch_zero(s, P.qcount, sizeof(uint32_t));
     ga.order = P.ccols;  // the sort leaves the ordered pairs in the first arrays
     ga.queue = P.qcount;
@@ -617,13 +674,35 @@ static void enqueue_gram(hipStream_t s, letkf_plan& P, int nblocks, bool queue) 
   launch_letkf_gram(s, ga);
 }
 
-static void enqueue_solve_transform(hipStream_t s, letkf_plan& P, const hipEvent_t* ev_stage) {
-  letkf_solve_args sa = P.sa;
-  for (int i = 0; i < 3; ++i) sa.ev_stage[i] = ev_stage ? ev_stage[i] : nullptr;
-  launch_zero(s, sa.status, sizeof(int));
-  launch_zero(s, sa.sweeps, sizeof(int) * P.ncol);
-  launch_letkf_solve(s, sa);
+// Event marks of one pass: gram start, gram end, E1, E2, E3 (tridiagonal
+// path), solve end, transform end.
+struct pass_marks {
+  hipEvent_t e[7];
+};
+
+// Gram -> solve -> transform over every pass of columns.  `marks` (optional)
+// receives one set of events per pass.
+static void enqueue_columns(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg,
+                            const std::vector<pass_marks>* marks) {
+  launch_zero(s, P.sa.status, sizeof(int));
+  launch_zero(s, P.sweeps, sizeof(int) * P.ncol);
+  int pass = 0;
+  for (int64_t c0 = 0; c0 < P.ncol; c0 += P.chunk, ++pass) {
+    const int64_t nc = std::min(P.chunk, P.ncol - c0);
+    const pass_marks* m = marks ? &(*marks)[pass] : nullptr;
+    if (m) MDA_HIP(hipEventRecord(m->e[0], s));
+    enqueue_gram(s, P, c0, nc, cfg.first, cfg.second != 0);
+    if (m) MDA_HIP(hipEventRecord(m->e[1], s));
+    letkf_solve_args sa = solve_pass(P, c0, nc);
+    for (int i = 0; i < 3; ++i) sa.ev_stage[i] = m ? m->e[2 + i] : nullptr;
+    launch_letkf_solve(s, sa);
+    if (m) MDA_HIP(hipEventRecord(m->e[5], s));
+    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, c0, nc);
+    if (m) MDA_HIP(hipEventRecord(m->e[6], s));
+  }
 }
+
+static int64_t n_passes(const letkf_plan& P) { return (P.ncol + P.chunk - 1) / P.chunk; }
 
 static void enqueue_stats(hipStream_t s, letkf_plan& P) {
   launch_zero(s, P.stats, sizeof(int64_t) * 9);
@@ -637,7 +716,9 @@ static void finish_letkf(mda_ctx* c, letkf_plan& P, mda_letkf_diag* diag) {
   int hstat = 0;
   d2h(c, hs, P.stats, sizeof(hs));
   d2h(c, &hstat, P.sa.status, sizeof(int));
-  c->last_T = P.sa.T; c->last_k = P.k; c->last_ncol = P.ncol; c->last_nloc = P.ga.nloc;
+  // the transforms stay readable only when one pass covered every column
+  c->last_T = P.chunk == P.ncol ? P.sa.T : nullptr;
+  c->last_k = P.k; c->last_ncol = P.ncol; c->last_nloc = P.ga.nloc;
   c->last_flag = P.eig_path ? P.sa.flag : nullptr;
   if (hstat) throw mda_error(MDA_E_NUMERIC, "LETKF: a local Pa is not positive definite");
   unstage_ensemble(c, P.ens, P.de);
@@ -658,31 +739,48 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     hipStream_t s = c->stream;
     MDA_HIP(hipEventRecord(c->ev[0], s));
     letkf_plan P = plan_letkf(c, prm, ens, geom, obs, hop, "", false);
+    const int64_t npass = n_passes(P);
+    while ((int64_t)c->evpool.size() < 7 * npass) {
+      hipEvent_t e = nullptr;
+      MDA_HIP(hipEventCreate(&e));
+      c->evpool.push_back(e);
+    }
+    std::vector<pass_marks> marks(npass);
+    for (int64_t i = 0; i < npass; ++i)
+      for (int j = 0; j < 7; ++j) marks[i].e[j] = c->evpool[7 * i + j];
     MDA_HIP(hipEventRecord(c->ev[1], s));
     enqueue_local(s, P);
     MDA_HIP(hipEventRecord(c->ev[2], s));
     enqueue_buckets(s, P);
     MDA_HIP(hipEventRecord(c->ev[3], s));
-    enqueue_gram(s, P, 0, false);
-    MDA_HIP(hipEventRecord(c->ev[4], s));
-    enqueue_solve_transform(s, P, &c->ev[7]);
-    MDA_HIP(hipEventRecord(c->ev[5], s));
-    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T);
+    enqueue_columns(s, P, {0, 0}, &marks);
     MDA_HIP(hipEventRecord(c->ev[6], s));
     enqueue_stats(s, P);
     finish_letkf(c, P, diag);
     if (diag) {
       diag->ms_obs = elapsed(c->ev[1], c->ev[2]);
       diag->ms_bucket = elapsed(c->ev[2], c->ev[3]);
-      diag->ms_gram = elapsed(c->ev[3], c->ev[4]);
-      diag->ms_solve = elapsed(c->ev[4], c->ev[5]);
-      diag->ms_transform = elapsed(c->ev[5], c->ev[6]);
       diag->ms_total = elapsed(c->ev[0], c->ev[6]);
+      double g = 0, so = 0, tr = 0, e1 = 0, e2 = 0, e3 = 0, jf = 0;
+      for (const pass_marks& m : marks) {
+        g += elapsed(m.e[0], m.e[1]);
+        so += elapsed(m.e[1], m.e[5]);
+        tr += elapsed(m.e[5], m.e[6]);
+        if (P.eig_path) {
+          e1 += elapsed(m.e[1], m.e[2]);
+          e2 += elapsed(m.e[2], m.e[3]);
+          e3 += elapsed(m.e[3], m.e[4]);
+          jf += elapsed(m.e[4], m.e[5]);
+        }
+      }
+      diag->ms_gram = g;
+      diag->ms_solve = so;
+      diag->ms_transform = tr;
       if (P.eig_path) {
-        diag->ms_eig_tridiag = elapsed(c->ev[4], c->ev[7]);
-        diag->ms_eig_values = elapsed(c->ev[7], c->ev[8]);
-        diag->ms_eig_vectors = elapsed(c->ev[8], c->ev[9]);
-        diag->ms_solve_jacobi = elapsed(c->ev[9], c->ev[5]);
+        diag->ms_eig_tridiag = e1;
+        diag->ms_eig_values = e2;
+        diag->ms_eig_vectors = e3;
+        diag->ms_solve_jacobi = jf;
       } else {
         diag->ms_eig_tridiag = diag->ms_eig_values = diag->ms_eig_vectors = 0.0;
         diag->ms_solve_jacobi = diag->ms_solve;
@@ -710,9 +808,11 @@ static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_q
   for (int q = 0; q <= (allow_queue ? 1 : 0); ++q) {
     for (int nb : cand_blocks) {
       if (q == 0 && nb != 0) continue;  // grid-stride mode: one wave per column
-      enqueue_gram(s, P, nb, q != 0);   // warm
+      // timed on the first pass of columns
+      const int64_t nc = std::min(P.chunk, P.ncol);
+      enqueue_gram(s, P, 0, nc, nb, q != 0);  // warm
       MDA_HIP(hipEventRecord(c->ev[10], s));
-      enqueue_gram(s, P, nb, q != 0);
+      enqueue_gram(s, P, 0, nc, nb, q != 0);
       MDA_HIP(hipEventRecord(c->ev[11], s));
       MDA_HIP(hipEventSynchronize(c->ev[11]));
       const float ms = elapsed(c->ev[10], c->ev[11]);
@@ -725,9 +825,7 @@ static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_q
 static void enqueue_plan(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg) {
   enqueue_local(s, P);
   enqueue_buckets(s, P);
-  enqueue_gram(s, P, cfg.first, cfg.second != 0);
-  enqueue_solve_transform(s, P, nullptr);
-  launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T);
+  enqueue_columns(s, P, cfg, nullptr);
   enqueue_stats(s, P);
 }
 
@@ -850,7 +948,9 @@ int mda_letkf_analyse_batch(mda_ctx* c, int32_t nb, const mda_letkf_params* prm,
 
 int mda_letkf_last_transforms(mda_ctx* c, const int64_t* cols, int64_t ncols, double* T_out) {
   return guarded(c, [&] {
-    MDA_REQUIRE(c->last_T != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(c->last_nloc != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(c->last_T != nullptr,
+                "the transforms of an analysis run in column passes are not kept");
     MDA_REQUIRE(cols && T_out, "cols and T_out are required");
     const size_t kk = (size_t)c->last_k * c->last_k;
     for (int64_t i = 0; i < ncols; ++i) {
@@ -863,7 +963,7 @@ int mda_letkf_last_transforms(mda_ctx* c, const int64_t* cols, int64_t ncols, do
 
 int mda_letkf_last_local_counts(mda_ctx* c, int32_t* counts) {
   return guarded(c, [&] {
-    MDA_REQUIRE(c->last_T != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(c->last_nloc != nullptr, "no LETKF analysis on this context yet");
     MDA_REQUIRE(counts, "counts is required");
     d2h(c, counts, c->last_nloc, sizeof(int32_t) * c->last_ncol);
   });
@@ -871,7 +971,7 @@ int mda_letkf_last_local_counts(mda_ctx* c, int32_t* counts) {
 
 int mda_letkf_last_column_flags(mda_ctx* c, int32_t* flags) {
   return guarded(c, [&] {
-    MDA_REQUIRE(c->last_T != nullptr, "no LETKF analysis on this context yet");
+    MDA_REQUIRE(c->last_nloc != nullptr, "no LETKF analysis on this context yet");
     MDA_REQUIRE(flags, "flags is required");
     if (c->last_flag)
       d2h(c, flags, c->last_flag, sizeof(int32_t) * c->last_ncol);

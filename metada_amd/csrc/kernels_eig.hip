@@ -626,6 +626,448 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
 }
 
 // ---------------------------------------------------------------------------
+// Ensembles wider than one wavefront (k = 128): the same three stages with
+// two wavefronts per column (one row / one eigenvalue per thread) and, for
+// the eigenvalues, bisection on Sturm counts.
+// ---------------------------------------------------------------------------
+
+// Sum over the block's two wavefronts; `red` holds one slot per wavefront.
+// Ends with a block barrier, so the caller may read shared values written
+// before the call.
+__device__ inline double block2_total(double v, double* red) {
+  const double s = wave_total(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return red[0] + red[1];
+}
+
+// E1 for N = 128: thread r keeps row r of A in registers.  Per step three
+// block barriers: |x|^2 with the pivot and diagonal, u, and p^T u.
+template <int N>
+__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
+void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__ nloc,
+             double rho, const double* __restrict__ rho_col, int prior,
+             double* __restrict__ de, double* __restrict__ Uout) {
+  static_assert(N == 128, "two wavefronts per column");
+  constexpr int n = N;
+  __shared__ double ub[N], wb[N], red[2][2], xs[2];
+  const int t = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
+    if (nloc[g] == 0) continue;
+    const double r = rho_col ? rho_col[g] : rho;
+    const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
+    const double* Cg = C + (size_t)g * n * n;
+    double ar[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) ar[c] = Cg[(size_t)t * n + c] + (t == c ? sigma : 0.0);
+    double* U = Uout + (size_t)g * n * n;
+    double* dg = de + (size_t)g * 2 * n;
+    double xc = ar[0];  // A[t][j] of the current step
+    constexpr int NCH = (N - 2 + 15) / 16;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      const int c0 = 16 * ch;
+      const int jend = (16 * ch + 16 < n - 2) ? 16 * ch + 16 : n - 2;
+#pragma unroll 1
+      for (int j = c0; j < jend; ++j) {
+        const double xi = t > j ? xc : 0.0;
+        if (t == j + 1) xs[0] = xc;
+        if (t == j) xs[1] = xc;
+        const double nrm2 = block2_total(xi * xi, red[0]);
+        const double x0 = xs[0];
+        const bool nz = nrm2 > 0.0;
+        const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
+        const double ui = nz ? xi - (t == j + 1 ? alpha : 0.0) : 0.0;
+        const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
+        if (t == 0) { dg[j] = xs[1]; dg[n + j] = alpha; }
+        U[(size_t)j * n + t] = t == 0 ? tau : ui;
+        ub[t] = ui;
+        __syncthreads();
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+        for (int c = c0; c < N; c += 4) {
+          p0 += ar[c] * ub[c];
+          p1 += ar[c + 1] * ub[c + 1];
+          p2 += ar[c + 2] * ub[c + 2];
+          p3 += ar[c + 3] * ub[c + 3];
+        }
+        const double p = tau * ((p0 + p1) + (p2 + p3));
+        const double K = 0.5 * tau * block2_total(p * ui, red[1]);
+        const double wi = t > j ? p - K * ui : 0.0;
+        wb[t] = wi;
+        __syncthreads();
+#pragma unroll
+        for (int c = c0; c < N; ++c) {
+          if ((c & 7) == 0) lds_reload_point();
+          ar[c] -= ui * wb[c] + wi * ub[c];
+          if (c > c0 && c <= c0 + 16 && c == j + 1) xc = ar[c];
+        }
+      }
+    }
+    // last 2x2 block straight from the owners' registers
+    if (t == n - 2) { dg[n - 2] = ar[n - 2]; dg[2 * n - 2] = ar[n - 1]; }
+    if (t == n - 1) { dg[n - 1] = ar[n - 1]; dg[2 * n - 1] = 0.0; }
+    __syncthreads();
+  }
+}
+
+// E2 by bisection on Sturm counts (one wavefront per column, lane j owns
+// eigenvalues j, j + 64, ...): count(x) = #{i : q_i < 0} with
+// q_0 = d_0 - x, q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}, |q_i| kept >= pivmin
+// (Golub & Van Loan, sec. 8.4.1).  Starts from the Gershgorin interval and
+// halves until the interval is within 2 eps of the eigenvalue.  Ascending
+// by construction.
+template <int N>
+__global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
+                                                            const int* __restrict__ nloc,
+                                                            const double* __restrict__ de,
+                                                            double* __restrict__ lam,
+                                                            int* __restrict__ flag) {
+  constexpr int n = N, NQ = (N + 63) / 64;
+  __shared__ double sd[4][N], se2[4][N];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t g = (int64_t)blockIdx.x * 4 + wv;
+  if (g >= ncol || nloc[g] == 0) return;  // whole wavefront; no block barriers below
+  const double* dg = de + (size_t)g * 2 * n;
+  double* d = sd[wv];
+  double* e2 = se2[wv];
+  for (int i = lane; i < n; i += 64) {
+    d[i] = dg[i];
+    const double e = i < n - 1 ? dg[n + i] : 0.0;
+    e2[i] = e * e;
+  }
+  wave_lds_sync();
+  double lo = 0.0, hi = 0.0, emax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = sqrt(e2[i]) + (i > 0 ? sqrt(e2[i - 1]) : 0.0);
+    lo = i == 0 ? d[i] - r : fmin(lo, d[i] - r);
+    hi = i == 0 ? d[i] + r : fmax(hi, d[i] + r);
+    emax = fmax(emax, e2[i]);
+  }
+  const double eps = 2.220446049250313e-16;
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+  lo -= 2.0 * eps * tnorm + pivmin;
+  hi += 2.0 * eps * tnorm + pivmin;
+  double a[NQ], b[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) { a[q] = lo; b[q] = hi; }
+  for (int it = 0; it < 128; ++it) {
+    bool open = false;
+    double x[NQ], qv[NQ];
+    int cnt[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      x[q] = 0.5 * (a[q] + b[q]);
+      open |= (lane + 64 * q < n) && b[q] - a[q] > 2.0 * eps * fmax(fabs(a[q]), fabs(b[q])) + pivmin
+              && x[q] != a[q] && x[q] != b[q];
+      qv[q] = 1.0;
+      cnt[q] = 0;
+    }
+    if (!wave_any(open)) break;
+    for (int i = 0; i < n; ++i) {
+      const double di = d[i], ei = i > 0 ? e2[i - 1] : 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        // 1/q by the hardware reciprocal and one Newton step
+        double rq = __builtin_amdgcn_rcp(qv[q]);
+        rq = fma(rq, fma(-qv[q], rq, 1.0), rq);
+        double v = (di - x[q]) - ei * rq;
+        if (fabs(v) < pivmin) v = -pivmin;
+        qv[q] = v;
+        cnt[q] += v < 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (cnt[q] > lane + 64 * q) b[q] = x[q];
+      else a[q] = x[q];
+    }
+  }
+  double* lg = lam + (size_t)g * n;
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int j = lane + 64 * q;
+    if (j < n) {
+      const double v = 0.5 * (a[q] + b[q]);
+      lg[j] = v;
+      bad |= !(fabs(v) <= 1.7e308);
+    }
+  }
+  if (wave_any(bad) && lane == 0) flag[g] = 1;
+}
+
+// E3 for N = 128: thread j owns eigenvalue j and keeps its eigenvector in
+// registers.  The twisted factorisation stores D+_i in z[i] on the way down
+// and, in a second bottom-up pass, U_i in z[i+1] for i >= twist (slots not
+// needed by the top part).  Eigenvectors go through the column's scratch Zg
+// ([component][vector]) for the near-pair (Lowdin) step and for
+// W = coef (f(sigma) I - U_s U_s^T), formed on FP64 MFMA by both wavefronts
+// (lower tiles dealt round-robin).  Householder rows are staged through LDS
+// 16 at a time.  Same checks and fallback flags as the one-wavefront kernel.
+template <int N>
+__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
+void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+                     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
+                     const double* __restrict__ rho_col, int prior, int combine, double null_rel,
+                     const double* __restrict__ Ug, double* __restrict__ Zs,
+                     double* __restrict__ T, int* __restrict__ flag) {
+  static_assert(N == 128, "two wavefronts per column");
+  constexpr int n = N;
+  constexpr int BAND = 8;
+  constexpr int RC = 16;  // Householder rows per staging chunk
+  __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N], sk[N];
+  __shared__ double dots[N][BAND];
+  __shared__ double ur[RC * N];
+  __shared__ int s_why, s_near;
+  const int t = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
+    if (nloc[g] == 0 || flag[g]) continue;  // empty columns / fallback columns
+    const double r = rho_col ? rho_col[g] : rho;
+    double* Tg = T + (size_t)g * n * n;
+    double* Zg = Zs + (size_t)g * n * n;
+    const double* dg = de + (size_t)g * 2 * n;
+    sd[t] = dg[t];
+    se[t] = t < n - 1 ? dg[n + t] : 0.0;
+    sl[t] = lam[(size_t)g * n + t];
+    sb[t] = bvec[(size_t)g * n + t];
+    if (t == 0) { s_why = 0; s_near = 0; }
+    __syncthreads();
+    double tnorm = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < N; ++i) tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+    const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
+    const double l = sl[t];
+    double z[N];
+    {
+      // top-down: z[i] = D+_i (safeguarded), L_i = e_i / D+_i
+      double dp = sd[0] - l;
+#pragma unroll
+      for (int i = 0; i < N - 1; ++i) {
+        if ((i & 7) == 0) lds_reload_point();
+        z[i] = safe_pivot(dp, tiny);
+        dp = sd[i + 1] - l - se[i] / z[i] * se[i];
+      }
+      z[N - 1] = dp;
+      double gbest = fabs(dp);
+      int kt = N - 1;
+      double dm = sd[N - 1] - l;
+#pragma unroll
+      for (int i = N - 2; i >= 0; --i) {
+        if ((i & 7) == 6) lds_reload_point();
+        const double u = se[i] / safe_pivot(dm, tiny);
+        dm = sd[i] - l - u * se[i];
+        const double gam = fabs(z[i] + dm - (sd[i] - l));
+        if (gam < gbest) { gbest = gam; kt = i; }
+      }
+      // U_i (i >= kt) into z[i+1], by the same bottom-up recurrence.  The
+      // shift goes through an empty asm so the compiler recomputes the
+      // recurrence instead of keeping all of the first pass's U_i live.
+      double lr = l;
+      asm volatile("" : "+v"(lr));
+      dm = sd[N - 1] - lr;
+#pragma unroll
+      for (int i = N - 2; i >= 0; --i) {
+        if ((i & 7) == 6) lds_reload_point();
+        const double u = se[i] / safe_pivot(dm, tiny);
+        dm = sd[i] - lr - u * se[i];
+        if (i >= kt) z[i + 1] = u;
+      }
+      double zc = 1.0;
+#pragma unroll
+      for (int i = N - 2; i >= 0; --i) {
+        if ((i & 7) == 6) lds_reload_point();
+        if (i < kt) { zc = -(se[i] / z[i]) * zc; z[i] = zc; }  // z_i = -L_i z_{i+1}
+      }
+      double nrm = 0.0;
+      zc = 1.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double v = i < kt ? z[i] : (i == kt ? 1.0 : -z[i] * zc);
+        if (i >= kt) zc = v;
+        z[i] = v;
+        nrm += v * v;
+      }
+      const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) z[i] *= inv;
+    }
+    const bool pri = prior != 0;
+    const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
+    const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
+    const bool skip = fabs(l - sigma) <= null_tol;
+    const double wj = skip ? 0.0 : 1.0 - sigma / l;
+    sy[t] = skip ? 1.0 : 0.0;  // skip flags, shared
+    swt[t] = wj;
+    int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
+    if (!skip) {
+      double res = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        if ((i & 7) == 0) lds_reload_point();
+        double v = (sd[i] - l) * z[i];
+        if (i > 0) v += se[i > 0 ? i - 1 : 0] * z[i > 0 ? i - 1 : 0];
+        if (i < N - 1) v += se[i] * z[i < N - 1 ? i + 1 : 0];
+        res = fmax(res, fabs(v));
+      }
+      if (!(res <= 1e-12 * tnorm)) why |= 2;
+      if (!(l >= sigma - null_tol)) why |= 8;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) Zg[(size_t)i * n + t] = z[i];
+    __threadfence_block();
+    __syncthreads();
+    // near pairs: symmetric (Lowdin) correction, then a check
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+      for (int q = 0; q < BAND; ++q) dots[t][q] = 0.0;
+      bool near = false;
+      if (!skip) {
+        for (int q = t + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
+          if (sy[q] != 0.0) continue;
+          near = true;
+          double dot = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) dot += z[i] * Zg[(size_t)i * n + q];
+          if (q - t > BAND) {
+            if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
+          } else {
+            dots[t][q - t - 1] = dot;
+            if (pass == 1 && fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
+          }
+        }
+      }
+      if (pass == 0 && near) atomicOr(&s_near, 1);
+      __syncthreads();
+      if (pass == 1 || s_near == 0) break;
+      if (!skip) {
+        for (int q = t - BAND; q <= t + BAND; ++q) {
+          if (q < 0 || q >= n || q == t) continue;
+          const double c = q > t ? dots[t][q - t - 1] : dots[q][t - q - 1];
+          if (c != 0.0) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Zg[(size_t)i * n + q];
+          }
+        }
+      }
+      __syncthreads();  // every read of the uncorrected vectors is done
+#pragma unroll
+      for (int i = 0; i < N; ++i) Zg[(size_t)i * n + t] = z[i];
+      __threadfence_block();
+      __syncthreads();
+    }
+    if (why) atomicOr(&s_why, why);
+    __syncthreads();
+    if (s_why != 0) {
+      if (t == 0) flag[g] = s_why;
+      __syncthreads();
+      continue;
+    }
+    // ---- z <- Q z, Householder rows staged through LDS ----
+    {
+      const double* Ucol = Ug + (size_t)g * n * n;
+      constexpr int NC = (N - 2 + RC - 1) / RC;
+#pragma unroll
+      for (int c = NC - 1; c >= 0; --c) {
+        const int jlo = RC * c;
+        const int jhi = (RC * c + RC - 1 < N - 3) ? RC * c + RC - 1 : N - 3;
+        __syncthreads();  // previous chunk's readers are done
+        for (int idx = t; idx < (jhi - jlo + 1) * N; idx += N) ur[idx] = Ucol[(size_t)jlo * N + idx];
+        __syncthreads();
+#pragma unroll 1
+        for (int j = jhi; j >= jlo; --j) {
+          const double* u = ur + (j - jlo) * N;
+          const double tau = u[0];
+          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+          for (int i = jlo + 1; i < N; ++i) {
+            if ((i & 15) == 0) lds_reload_point();
+            const double tt = u[i] * z[i];
+            switch (i & 3) {
+              case 0: p0 += tt; break;
+              case 1: p1 += tt; break;
+              case 2: p2 += tt; break;
+              default: p3 += tt; break;
+            }
+          }
+          const double dot = tau * ((p0 + p1) + (p2 + p3));
+#pragma unroll
+          for (int i = jlo + 1; i < N; ++i) {
+            if ((i & 15) == 0) lds_reload_point();
+            z[i] -= dot * u[i];
+          }
+        }
+      }
+    }
+    // ---- U_s = Z diag(f), y' for wbar ----
+    const double sc = pri ? 1.0 : r;
+    const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
+    double yb = 0.0;
+    if (!skip) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) yb += z[i] * sb[i];
+      yb *= (1.0 / sigma - 1.0 / l);
+    }
+    sy[t] = f > 0.0 ? yb / f : 0.0;
+    double cs = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      Zg[(size_t)i * n + t] = z[i] * f;
+      cs += z[i] * f;
+    }
+    sk[t] = cs;
+    __threadfence_block();
+    __syncthreads();
+    const double coef = sqrt((double)(n - 1)) * sqrt(sc);
+    const double fs = 1.0 / sqrt(sigma);
+    // wbar_t = s [ b_t / sigma - sum_j U_s[t][j] y'_j ]
+    double acc = 0.0, acs = 0.0;
+    for (int e2 = 0; e2 < N; ++e2) {
+      const double us = Zg[(size_t)t * n + e2];
+      acc += us * sy[e2];
+      acs += us * sk[e2];
+    }
+    const double wbar = sc * (sb[t] / sigma - acc);
+    if (combine == MDA_COMBINE_ELEMENTWISE) {
+      // t_i = wbar_i + coef (f(sigma) - sum_a U_s[i][a] (sum_j U_s[j][a]))
+      for (int idx = t; idx < n * n; idx += N) Tg[idx] = 0.0;
+      __syncthreads();
+      Tg[(size_t)t * n + t] = wbar + coef * (fs - acs);
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();  // sy (y') reads done
+    sy[t] = wbar;
+    __syncthreads();
+    // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA ----
+    constexpr int nt = N / 16;
+    const int lane = t & 63, wv = t >> 6;
+    const int i16 = lane & 15, kk = lane >> 4;
+    int tile = 0;
+    for (int ti = 0; ti < nt; ++ti) {
+      for (int tj = 0; tj <= ti; ++tj, ++tile) {
+        if ((tile & 1) != wv) continue;
+        d4 am = {0.0, 0.0, 0.0, 0.0};
+        const double* ra = Zg + (size_t)(16 * ti + i16) * n + kk;
+        const double* rb = Zg + (size_t)(16 * tj + i16) * n + kk;
+#pragma unroll 8
+        for (int s4 = 0; s4 < N / 4; ++s4)
+          am = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[4 * s4], rb[4 * s4], am, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
+          const double w = coef * ((row == col ? fs : 0.0) - am[q]);
+          Tg[(size_t)row * n + col] = sy[row] + w;
+          if (ti != tj) Tg[(size_t)col * n + row] = sy[col] + w;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 template <int N>
 static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                          int* flag) {
@@ -646,9 +1088,32 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }
 
+// k = 128: two wavefronts per column for E1 / E3, bisection for E2; the
+// eigenvectors pass through the scratch a.Zs ([ncol][k][k])
+template <int N>
+static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
+                         int* flag) {
+  MDA_REQUIRE(a.Zs != nullptr, "eigenvector scratch missing");
+  const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
+  launch_zero(s, flag, sizeof(int) * a.ncol);
+  const unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
+  hipLaunchKernelGGL(k_trd_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
+                     prior, de, a.U);
+  MDA_LAUNCH_CHECK();
+  if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
+  hipLaunchKernelGGL(k_trd_eigvals_bisect<N>, dim3((unsigned)((a.ncol + 3) / 4)), dim3(256), 0,
+                     s, a.ncol, a.nloc, de, lam, flag);
+  MDA_LAUNCH_CHECK();
+  if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
+  hipLaunchKernelGGL(k_trd_vectors_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.nloc, de, lam, a.b,
+                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.Zs, a.T, flag);
+  MDA_LAUNCH_CHECK();
+  if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
+}
+
 bool letkf_eig_supported(int k) {
   switch (k) {
-    case 8: case 16: case 20: case 32: case 40: case 48: case 64: return true;
+    case 8: case 16: case 20: case 32: case 40: case 48: case 64: case 128: return true;
     default: return false;
   }
 }
@@ -663,6 +1128,7 @@ void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, doub
     case 40: launch_eig_n<40>(s, a, de, lam, flag); break;
     case 48: launch_eig_n<48>(s, a, de, lam, flag); break;
     case 64: launch_eig_n<64>(s, a, de, lam, flag); break;
+    case 128: launch_eig_w<128>(s, a, de, lam, flag); break;
     default: throw mda_error(MDA_E_UNSUPPORTED, "tridiagonal eigensolver: no kernel for this k");
   }
 }

@@ -266,7 +266,7 @@ __device__ inline col_window column_window(const letkf_gram_args& a, int64_t g) 
   col_window w;
   w.cc = 0.0;
   if (a.col_a) { w.ca = a.col_a[g]; w.cb = a.col_b[g]; }
-  else { w.ca = (double)(g % a.nx); w.cb = (double)(g / a.nx); }
+  else { w.ca = (double)((g + a.col0) % a.nx); w.cb = (double)((g + a.col0) / a.nx); }
   if (a.col_c) w.cc = a.col_c[g];
   double r_lo, r_hi, c_lo, c_hi;
   bool full = false;
@@ -326,9 +326,13 @@ void launch_column_cost(hipStream_t s, const letkf_gram_args& a, uint32_t* keys,
   MDA_LAUNCH_CHECK();
 }
 
-template <int KT>
+// NPART > 1 (k > 64): the lower tiles are dealt round-robin to NPART waves
+// per column (tile t to wave t % NPART), each repeating the candidate search,
+// so the accumulators of one wave stay within its registers.
+template <int KT, int NPART>
 __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   constexpr int NT = KT * (KT + 1) / 2;
+  constexpr int NA = (NT + NPART - 1) / NPART;
   __shared__ int q_off[4][64];
   __shared__ double q_w[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -336,12 +340,14 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   const int k = a.k;
   const bucket_grid& G = a.grid;
   const double cgc = a.radius / 2.0;
+  const int64_t nitems = a.ncol * NPART;
   int64_t it = a.queue ? queue_next(a.queue) : (int64_t)blockIdx.x * 4 + wv;
-  for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + (int64_t)gridDim.x * 4) {
-    const int64_t g = a.order ? (int64_t)a.order[it] : it;
-    d4 acc[NT];
+  for (; it < nitems; it = a.queue ? queue_next(a.queue) : it + (int64_t)gridDim.x * 4) {
+    const int part = (int)(it % NPART);
+    const int64_t g = a.order ? (int64_t)a.order[it / NPART] : it / NPART;
+    d4 acc[NA];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < NA; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
     double bp[KT];
 #pragma unroll
     for (int t = 0; t < KT; ++t) bp[t] = 0.0;
@@ -401,7 +407,9 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
               bp[ta] += aw * dv;
 #pragma unroll
               for (int tb = 0; tb <= ta; ++tb) {
-                acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(aw, yv[tb], acc[tile], 0, 0, 0);
+                if (tile % NPART == part)
+                  acc[tile / NPART] =
+                      __builtin_amdgcn_mfma_f64_16x16x4f64(aw, yv[tb], acc[tile / NPART], 0, 0, 0);
                 ++tile;
               }
             }
@@ -422,17 +430,20 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
     for (int ta = 0; ta < KT; ++ta) {
 #pragma unroll
       for (int tb = 0; tb <= ta; ++tb) {
+        if (tile % NPART == part) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int row = 16 * ta + kk + 4 * r, col = 16 * tb + i16;
-          if (row < k && col < k) {
-            Cg[(size_t)row * k + col] = acc[tile][r];
-            if (ta != tb) Cg[(size_t)col * k + row] = acc[tile][r];
+          for (int r = 0; r < 4; ++r) {
+            int row = 16 * ta + kk + 4 * r, col = 16 * tb + i16;
+            if (row < k && col < k) {
+              Cg[(size_t)row * k + col] = acc[tile / NPART][r];
+              if (ta != tb) Cg[(size_t)col * k + row] = acc[tile / NPART][r];
+            }
           }
         }
         ++tile;
       }
     }
+    if (part != 0) continue;  // b and the local count: first wave of the column
     if (kk == 0) {
 #pragma unroll
       for (int ta = 0; ta < KT; ++ta) {
@@ -902,10 +913,13 @@ void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, doub
 }
 
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
-  unsigned nb = (unsigned)((a.ncol + 3) / 4);
+  MDA_REQUIRE(a.k >= 1 && a.k <= 128, "LETKF Gram supports k <= 128");
+  const int npart = a.k > 64 ? 2 : 1;  // waves per column (FP64 Gram)
+  unsigned nb = (unsigned)((a.ncol * (a.precision == MDA_PREC_FP64 ? npart : 1) + 3) / 4);
   if (nb > 262144) nb = 262144;
   if (a.nblocks > 0 && (unsigned)a.nblocks < nb) nb = (unsigned)a.nblocks;
-  MDA_REQUIRE(!a.queue || a.ncol < ((int64_t)1 << 32) - 262144 * 4, "too many columns for the queue");
+  MDA_REQUIRE(!a.queue || a.ncol * npart < ((int64_t)1 << 32) - 262144 * 4,
+              "too many columns for the queue");
   int kt = (a.k + 15) / 16;
   if (a.precision != MDA_PREC_FP64) {
     MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "mixed Gram inputs missing");
@@ -920,18 +934,22 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
       case 2: MDA_GM(2); break;
       case 3: MDA_GM(3); break;
       case 4: MDA_GM(4); break;
-      default: throw mda_error(MDA_E_UNSUPPORTED, "LETKF Gram supports k <= 64 in this build");
+      default:
+        throw mda_error(MDA_E_UNSUPPORTED,
+                        "FP32 / BF16x3 Gram supports k <= 64 in this build (use FP64 for k > 64)");
     }
 #undef MDA_GM
     MDA_LAUNCH_CHECK();
     return;
   }
   switch (kt) {
-    case 1: hipLaunchKernelGGL(k_letkf_gram_f64<1>, dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_letkf_gram_f64<2>, dim3(nb), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_letkf_gram_f64<3>, dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_letkf_gram_f64<4>, dim3(nb), dim3(256), 0, s, a); break;
-    default: throw mda_error(MDA_E_UNSUPPORTED, "LETKF Gram supports k <= 64 in this build");
+    case 1: hipLaunchKernelGGL((k_letkf_gram_f64<1, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_letkf_gram_f64<2, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_letkf_gram_f64<3, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_letkf_gram_f64<4, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    default:  // 64 < k <= 128: 8 tile rows, two waves per column
+      hipLaunchKernelGGL((k_letkf_gram_f64<8, 2>), dim3(nb), dim3(256), 0, s, a);
+      break;
   }
   MDA_LAUNCH_CHECK();
 }
@@ -1066,7 +1084,11 @@ void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a) {
   unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
   if (nb < 1) nb = 1;
   double* gscratch = nullptr;
-  if (!use_lds) throw mda_error(MDA_E_UNSUPPORTED, "LETKF solve for k > 96 not in this build");
+  if (!use_lds) {  // A and V in a global per-block scratch
+    MDA_REQUIRE(a.jscratch && a.jblocks > 0, "Jacobi scratch missing");
+    gscratch = a.jscratch;
+    if (nb > (unsigned)a.jblocks) nb = (unsigned)a.jblocks;
+  }
   const int* only = nullptr;
   if (a.sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(a.k) && a.method == 0) {
     // tridiagonal eigensolver; flagged columns fall back to Jacobi below
@@ -1088,7 +1110,8 @@ constexpr int TR_C = 16, TR_L = 8;
 
 __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X, int k,
                                                          int64_t ncol, int64_t nlev,
-                                                         const double* __restrict__ T) {
+                                                         const double* __restrict__ T,
+                                                         int64_t col0, int64_t nc) {
   extern __shared__ double xs[];  // [TR_L][TR_C][k+1]
   __shared__ double xm[TR_L][TR_C];
   const int tid = threadIdx.x, nt = blockDim.x, ld = k + 1;
@@ -1099,13 +1122,16 @@ __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X,
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
   const int64_t nlt = (nlev + TR_L - 1) / TR_L;
-  if (vb >= ((ncol + TR_C - 1) / TR_C) * nlt) return;
+  if (vb >= ((nc + TR_C - 1) / TR_C) * nlt) return;
+  // g0: first column of the tile within the chunk [col0, col0 + nc); T is
+  // indexed within the chunk, X by the global column
   const int64_t g0 = (vb / nlt) * TR_C, l0 = (vb % nlt) * TR_L;
-  const int ncl = (int)(ncol - g0 < TR_C ? ncol - g0 : TR_C);
+  const int ncl = (int)(nc - g0 < TR_C ? nc - g0 : TR_C);
   const int nll = (int)(nlev - l0 < TR_L ? nlev - l0 : TR_L);
   for (int idx = tid; idx < k * TR_L * TR_C; idx += nt) {
     int c = idx % TR_C, l = (idx / TR_C) % TR_L, m = idx / (TR_C * TR_L);
-    if (c < ncl && l < nll) xs[(l * TR_C + c) * ld + m] = X[(size_t)m * n + (l0 + l) * ncol + g0 + c];
+    if (c < ncl && l < nll)
+      xs[(l * TR_C + c) * ld + m] = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c];
   }
   __syncthreads();
   const double rk = 1.0 / (double)k;
@@ -1156,7 +1182,8 @@ __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X,
   __syncthreads();
   for (int idx = tid; idx < k * TR_L * TR_C; idx += nt) {
     int c = idx % TR_C, l = (idx / TR_C) % TR_L, m = idx / (TR_C * TR_L);
-    if (c < ncl && l < nll) X[(size_t)m * n + (l0 + l) * ncol + g0 + c] = xs[(l * TR_C + c) * ld + m];
+    if (c < ncl && l < nll)
+      X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c] = xs[(l * TR_C + c) * ld + m];
   }
 }
 
@@ -1175,7 +1202,8 @@ constexpr int XT_C = 4, XT_L = 20;
 template <int K>
 __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict__ X,
                                                              int64_t ncol, int64_t nlev,
-                                                             const double* __restrict__ T) {
+                                                             const double* __restrict__ T,
+                                                             int64_t col0, int64_t nc) {
   constexpr int KP = K + 2;  // row pad: 16-byte aligned, banks shift by 4 per row
   extern __shared__ double xs[];  // [XT_L][XT_C][KP]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1183,9 +1211,9 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
   const int64_t n = ncol * nlev;
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
-  const int64_t g0 = vb * XT_C;
-  if (g0 >= ncol) return;
-  const int ncl = (int)(ncol - g0 < XT_C ? ncol - g0 : XT_C);
+  const int64_t g0 = vb * XT_C;  // within the chunk [col0, col0 + nc)
+  if (g0 >= nc) return;
+  const int ncl = (int)(nc - g0 < XT_C ? nc - g0 : XT_C);
   const bool own = j < K && c < ncl;
   double t[K];
   double colsum = 0.0;
@@ -1205,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
     for (int idx = tid; idx < K * XT_L * XT_C; idx += 256) {
       const int cc = idx % XT_C, l = (idx / XT_C) % XT_L, m = idx / (XT_C * XT_L);
       if (cc < ncl && l < nll)
-        xs[(l * XT_C + cc) * KP + m] = X[(size_t)m * n + (l0 + l) * ncol + g0 + cc];
+        xs[(l * XT_C + cc) * KP + m] = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + cc];
     }
     __syncthreads();
     if (own) {
@@ -1221,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
             s1 += row[m + 1 < K ? m + 1 : m];
           }
         }
-        X[(size_t)j * n + (l0 + l) * ncol + g0 + c] = (s0 + s1) * rk * cmean + (a0 + a1);
+        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = (s0 + s1) * rk * cmean + (a0 + a1);
       }
     }
   }
@@ -1235,15 +1263,19 @@ bool letkf_transform_reg_supported(int k) {
 }
 
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T) {
+                            const double* T, int64_t col0, int64_t nc) {
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
+  if (nc < 0) nc = ncol;
+  MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
+  if (nc == 0) return;
   if (letkf_transform_reg_supported(k)) {
-    MDA_REQUIRE((ncol + XT_C - 1) / XT_C < (int64_t)1 << 31, "transform grid too large");
-    const unsigned nbr = (unsigned)(((ncol + XT_C - 1) / XT_C + 7) / 8 * 8);
+    MDA_REQUIRE((nc + XT_C - 1) / XT_C < (int64_t)1 << 31, "transform grid too large");
+    const unsigned nbr = (unsigned)(((nc + XT_C - 1) / XT_C + 7) / 8 * 8);
     const size_t shr = sizeof(double) * XT_L * XT_C * (k + 2);
 #define MDA_XT(KK)                                                                          \
   case KK:                                                                                 \
-    hipLaunchKernelGGL(k_letkf_transform_reg<KK>, dim3(nbr), dim3(256), shr, s, X, ncol, nlev, T); \
+    hipLaunchKernelGGL(k_letkf_transform_reg<KK>, dim3(nbr), dim3(256), shr, s, X, ncol, nlev, T, \
+                       col0, nc);                                                            \
     break;
     switch (k) { MDA_XT(8) MDA_XT(16) MDA_XT(20) MDA_XT(32) MDA_XT(40) MDA_XT(48) MDA_XT(64) }
 #undef MDA_XT
@@ -1251,10 +1283,11 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
     return;
   }
   size_t shm = sizeof(double) * TR_L * TR_C * (k + 1);
-  const int64_t tiles = ((ncol + TR_C - 1) / TR_C) * ((nlev + TR_L - 1) / TR_L);
+  const int64_t tiles = ((nc + TR_C - 1) / TR_C) * ((nlev + TR_L - 1) / TR_L);
   MDA_REQUIRE(tiles < (int64_t)1 << 31, "transform grid too large");
   const unsigned nb = (unsigned)((tiles + 7) / 8 * 8);
-  hipLaunchKernelGGL(k_letkf_transform, dim3(nb), dim3(256), shm, s, X, k, ncol, nlev, T);
+  hipLaunchKernelGGL(k_letkf_transform, dim3(nb), dim3(256), shm, s, X, k, ncol, nlev, T, col0,
+                     nc);
   MDA_LAUNCH_CHECK();
 }
 

@@ -53,6 +53,8 @@ struct mda_ctx {
   // autotuned Gram launch (blocks, work queue) by workload key
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> fj;
+  // timing events of column-chunked analyses (grown on demand)
+  std::vector<hipEvent_t> evpool;
   std::map<std::string, hipGraphExec_t> graphs;
   std::map<std::string, std::pair<int, int>> tune;
   // state of the last LETKF call (for mda_letkf_last_transforms)
@@ -222,6 +224,10 @@ struct letkf_gram_args {
   const uint32_t* order;
   uint32_t* queue;
   int nblocks;  // launch size (0: one wave per column)
+  // Global index of column 0.  The per-column arrays (col_a/b/c, C, b, nloc)
+  // start at that column; col0 only enters where the global index itself
+  // matters (grid coordinates of a GRID geometry without coordinates).
+  int64_t col0;
 };
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
 // per-column cost estimate (observations in the candidate buckets) as
@@ -260,6 +266,11 @@ struct letkf_solve_args {
   double* lam;   // [ncol][k]
   double* U;     // [ncol][k][k] Householder reflectors
   int* flag;     // [ncol] columns re-solved by the Jacobi fallback
+  double* Zs;    // [ncol][k][k] eigenvector scratch (k > 64)
+  // Jacobi in global memory (k too large for LDS): per-block scratch of
+  // 2 kp (kp + 1) doubles for jblocks blocks
+  double* jscratch;
+  int jblocks;
   int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
   hipEvent_t ev_stage[3];  // recorded after E1, E2, E3 when set (timing)
   // Reduced-precision Gram: eigenvalues within null_rel * ||A|| of (k-1)/rho
@@ -271,8 +282,10 @@ void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
 bool letkf_eig_supported(int k);
 void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                       int* flag);
+// X_a = xbar + X' T for the columns [col0, col0 + nc) of X (ncol columns in
+// all; nc < 0: every column); T holds the nc columns' transforms
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T);
+                            const double* T, int64_t col0 = 0, int64_t nc = -1);
 void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,
                        int64_t* out_empty);
 

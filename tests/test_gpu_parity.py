@@ -342,3 +342,70 @@ def test_letkf_mixed_precision_contractions(mda, ctx, orc, prec, bound, k):
     # the increment itself is reproduced, not just the background
     inc = np.sqrt(np.mean((Xo - case["X"]) ** 2))
     assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 10 * bound * inc
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_letkf_column_passes_match_single_pass(mda, ctx, orc, prec, monkeypatch):
+    """Gram -> solve -> transform over passes of 333 columns (the last one
+    short) reproduces the single-pass analysis bit for bit: every column is
+    computed from the same inputs whatever pass it falls in."""
+    code = {"fp64": mda._lib.PREC_FP64, "bf16x3": mda._lib.PREC_BF16X3}[prec]
+    case = latlon_case(k=40, p=4000, seed=91, cluster=True)
+    out = []
+    for chunk in (None, "333"):
+        if chunk is None:
+            monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+        else:
+            monkeypatch.setenv("MDA_LETKF_CHUNK_COLS", chunk)
+        ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+        obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                              coord=mda._lib.COORD_GEOGRAPHIC)
+        geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+        f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                       "localization_radius": 1500.0}),
+                      geometry=geom, ctx=ctx, precision=code)
+        diag = f.Analyse()
+        out.append((ens.data.copy(), diag))
+    monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for key in ("total_local_obs", "max_local_obs", "n_empty_columns", "n_fallback_columns"):
+        assert out[0][1][key] == out[1][1][key]
+    assert out[1][1]["ms_gram"] > 0 and out[1][1]["ms_transform"] > 0
+
+
+@pytest.fixture(scope="module")
+def k128_case(orc):
+    """k = 128 on a 32 x 16 x 3 grid: local counts 97..152, so about half the
+    local Gram matrices are rank-deficient; the oracle result is shared."""
+    case = latlon_case(nlon=32, nlat=16, nlev=3, k=128, p=3200, seed=128, nmodes=96)
+    prob = orc.Problem(128, case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], 1.0499999523162842, 2500.0)
+    Xo, _, nloc = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    return case, Xo, nloc
+
+
+@pytest.mark.parametrize("method", ["auto", "jacobi"])
+def test_letkf_k128_matches_oracle(mda, ctx, k128_case, method, monkeypatch):
+    """k = 128 (two wavefronts per column, bisection eigenvalues; or the
+    Jacobi kernel in global scratch) against the oracle, run in passes of
+    200 columns."""
+    case, Xo, nloc = k128_case
+    assert nloc.min() < 127 < nloc.max()
+    monkeypatch.setenv("MDA_LETKF_CHUNK_COLS", "200")
+    ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                          coord=mda._lib.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+    em = mda._lib.EIG_AUTO if method == "auto" else mda._lib.EIG_JACOBI
+    f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                   "localization_radius": 2500.0}),
+                  geometry=geom, ctx=ctx, eig_method=em)
+    assert f.inflation == 1.0499999523162842
+    diag = f.Analyse()
+    monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+    assert diag["total_local_obs"] == nloc.sum()
+    assert rel_err(ens.data, Xo) < TOL
+    if method == "auto":
+        assert diag["n_fallback_columns"] == 0
+        assert diag["ms_eig_values"] > 0
