@@ -587,7 +587,6 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
                prm->eig_method == MDA_EIG_AUTO;
   sa.U = P.eig_path ? c->get<double>(pre + "eig_U", (size_t)nck * k * k) : nullptr;
   sa.flag = c->get<int>(pre + "eig_flag", ncol);
-  sa.Zs = P.eig_path && k > 64 ? c->get<double>(pre + "eig_Z", (size_t)nck * k * k) : nullptr;
   // Jacobi (fallback, or the Jacobi-only method) beyond the LDS capacity runs
   // in a global per-block scratch
   {

@@ -801,32 +801,38 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
 // E3 for N = 128: thread j owns eigenvalue j and keeps its eigenvector in
 // registers.  The twisted factorisation stores D+_i in z[i] on the way down
 // and, in a second bottom-up pass, U_i in z[i+1] for i >= twist (slots not
-// needed by the top part).  Eigenvectors go through the column's scratch Zg
-// ([component][vector]) for the near-pair (Lowdin) step and for
+// needed by the top part).  Eigenvectors go through LDS (Z[component][vector],
+// row stride N + 1) for the near-pair (Lowdin) step and for
 // W = coef (f(sigma) I - U_s U_s^T), formed on FP64 MFMA by both wavefronts
-// (lower tiles dealt round-robin).  Householder rows are staged through LDS
-// 16 at a time.  Same checks and fallback flags as the one-wavefront kernel.
+// (lower tiles dealt round-robin); one column per CU (129 KB of LDS).
+// Householder rows are staged through LDS 8 at a time.  Same checks and
+// fallback flags as the one-wavefront kernel.
 template <int N>
 __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
 void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
                      const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
                      const double* __restrict__ rho_col, int prior, int combine, double null_rel,
-                     const double* __restrict__ Ug, double* __restrict__ Zs,
-                     double* __restrict__ T, int* __restrict__ flag) {
+                     const double* __restrict__ Ug, double* __restrict__ T,
+                     int* __restrict__ flag) {
   static_assert(N == 128, "two wavefronts per column");
   constexpr int n = N;
+  constexpr int ld = N + 1;
   constexpr int BAND = 8;
-  constexpr int RC = 16;  // Householder rows per staging chunk
+  constexpr int RC = 8;  // Householder rows per staging chunk
+  __shared__ double Z[N * ld];
   __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N], sk[N];
   __shared__ double dots[N][BAND];
   __shared__ double ur[RC * N];
   __shared__ int s_why, s_near;
-  const int t = threadIdx.x;
   for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
+    // the thread index is re-read per column through an empty asm: addresses
+    // derived from it are then formed where they are used instead of being
+    // hoisted out of the column loop (several hundred live registers)
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
     if (nloc[g] == 0 || flag[g]) continue;  // empty columns / fallback columns
     const double r = rho_col ? rho_col[g] : rho;
     double* Tg = T + (size_t)g * n * n;
-    double* Zg = Zs + (size_t)g * n * n;
     const double* dg = de + (size_t)g * 2 * n;
     sd[t] = dg[t];
     se[t] = t < n - 1 ? dg[n + t] : 0.0;
@@ -915,8 +921,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
       if (!(l >= sigma - null_tol)) why |= 8;
     }
 #pragma unroll
-    for (int i = 0; i < N; ++i) Zg[(size_t)i * n + t] = z[i];
-    __threadfence_block();
+    for (int i = 0; i < N; ++i) Z[i * ld + t] = z[i];
     __syncthreads();
     // near pairs: symmetric (Lowdin) correction, then a check
     for (int pass = 0; pass < 2; ++pass) {
@@ -929,7 +934,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
           near = true;
           double dot = 0.0;
 #pragma unroll
-          for (int i = 0; i < N; ++i) dot += z[i] * Zg[(size_t)i * n + q];
+          for (int i = 0; i < N; ++i) dot += z[i] * Z[i * ld + q];
           if (q - t > BAND) {
             if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
           } else {
@@ -947,14 +952,13 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
           const double c = q > t ? dots[t][q - t - 1] : dots[q][t - q - 1];
           if (c != 0.0) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Zg[(size_t)i * n + q];
+            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Z[i * ld + q];
           }
         }
       }
       __syncthreads();  // every read of the uncorrected vectors is done
 #pragma unroll
-      for (int i = 0; i < N; ++i) Zg[(size_t)i * n + t] = z[i];
-      __threadfence_block();
+      for (int i = 0; i < N; ++i) Z[i * ld + t] = z[i];
       __syncthreads();
     }
     if (why) atomicOr(&s_why, why);
@@ -1013,18 +1017,17 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
     double cs = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      Zg[(size_t)i * n + t] = z[i] * f;
+      Z[i * ld + t] = z[i] * f;
       cs += z[i] * f;
     }
     sk[t] = cs;
-    __threadfence_block();
     __syncthreads();
     const double coef = sqrt((double)(n - 1)) * sqrt(sc);
     const double fs = 1.0 / sqrt(sigma);
     // wbar_t = s [ b_t / sigma - sum_j U_s[t][j] y'_j ]
     double acc = 0.0, acs = 0.0;
     for (int e2 = 0; e2 < N; ++e2) {
-      const double us = Zg[(size_t)t * n + e2];
+      const double us = Z[t * ld + e2];
       acc += us * sy[e2];
       acs += us * sk[e2];
     }
@@ -1049,8 +1052,8 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
       for (int tj = 0; tj <= ti; ++tj, ++tile) {
         if ((tile & 1) != wv) continue;
         d4 am = {0.0, 0.0, 0.0, 0.0};
-        const double* ra = Zg + (size_t)(16 * ti + i16) * n + kk;
-        const double* rb = Zg + (size_t)(16 * tj + i16) * n + kk;
+        const double* ra = Z + (16 * ti + i16) * ld + kk;
+        const double* rb = Z + (16 * tj + i16) * ld + kk;
 #pragma unroll 8
         for (int s4 = 0; s4 < N / 4; ++s4)
           am = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[4 * s4], rb[4 * s4], am, 0, 0, 0);
@@ -1088,12 +1091,10 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }
 
-// k = 128: two wavefronts per column for E1 / E3, bisection for E2; the
-// eigenvectors pass through the scratch a.Zs ([ncol][k][k])
+// k = 128: two wavefronts per column for E1 / E3, bisection for E2
 template <int N>
 static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                          int* flag) {
-  MDA_REQUIRE(a.Zs != nullptr, "eigenvector scratch missing");
   const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
   launch_zero(s, flag, sizeof(int) * a.ncol);
   const unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
@@ -1106,7 +1107,7 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
   hipLaunchKernelGGL(k_trd_vectors_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.nloc, de, lam, a.b,
-                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.Zs, a.T, flag);
+                     a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.T, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }

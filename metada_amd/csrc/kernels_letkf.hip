@@ -1255,6 +1255,95 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
   }
 }
 
+// 4''. X_a = xbar (1 - colsum) + X' T as one FP64 MFMA product per column:
+// with T'[m][j] = T[m][j] + (1 - sum_m T[m][j]) / K,
+//   X_a[lev][j] = sum_m X[lev][m] T'[m][j]
+// (xbar = sum_m X[lev][m] / K), i.e. X_a = X T'.  A block covers XM_C = 4
+// consecutive columns, one wavefront each; the wavefront keeps T' of its
+// column in registers in the MFMA B layout (B[l/16][l%16], K/16 tiles x K/4
+// steps) and runs v_mfma_f64_16x16x4f64 over 16-level chunks.  X is staged
+// through LDS as [column][member][16 levels] from 32-byte column segments
+// (coalesced across the block's columns), and X_a leaves the same way.
+constexpr int XM_C = 4, XM_L = 16;
+
+template <int K>
+__global__ __launch_bounds__(256) void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
+                                                              int64_t nlev,
+                                                              const double* __restrict__ T,
+                                                              int64_t col0, int64_t nc) {
+  static_assert(K % 16 == 0 && K <= 64, "K must be a multiple of 16, at most 64");
+  constexpr int NTJ = K / 16, NS = K / 4;
+  __shared__ double xs[XM_C * K * XM_L];  // [column][member or j][level]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const int64_t n = ncol * nlev;
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t g0 = vb * XM_C;  // within the pass [col0, col0 + nc)
+  if (g0 >= nc) return;
+  const int ncl = (int)(nc - g0 < XM_C ? nc - g0 : XM_C);
+  const bool own = wv < ncl;
+  // T' of column g0 + wv in the B layout: tb[s][tj] = T'[4s + kk][16 tj + i16]
+  double tb[NS][NTJ];
+  {
+    const double* Tg = T + (size_t)(g0 + (own ? wv : 0)) * K * K;
+    double cs[NTJ];
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj) cs[tj] = 0.0;
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4)
+#pragma unroll
+      for (int tj = 0; tj < NTJ; ++tj) {
+        tb[s4][tj] = own ? Tg[(size_t)(4 * s4 + kk) * K + 16 * tj + i16] : 0.0;
+        cs[tj] += tb[s4][tj];
+      }
+    const double rk = 1.0 / (double)K;
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj) {
+      cs[tj] += __shfl_xor(cs[tj], 16);
+      cs[tj] += __shfl_xor(cs[tj], 32);
+      const double add = (1.0 - cs[tj]) * rk;
+#pragma unroll
+      for (int s4 = 0; s4 < NS; ++s4) tb[s4][tj] += add;
+    }
+  }
+  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
+    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
+    __syncthreads();  // the previous chunk's output has left xs
+    // stage X[m][l0 + l][g0 + c] -> xs[c][m][l]; consecutive threads take
+    // consecutive columns of one (member, level) row segment
+    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
+      double v = 0.0;
+      if (c < ncl && l < nll) v = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c];
+      xs[(c * K + m) * XM_L + l] = v;
+    }
+    __syncthreads();
+    d4 acc[NTJ];
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
+    const double* xa = xs + (wv * K + kk) * XM_L + i16;  // A[i16][kk] of step 0
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4) {
+      const double av = xa[4 * s4 * XM_L];
+#pragma unroll
+      for (int tj = 0; tj < NTJ; ++tj)
+        acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tb[s4][tj], acc[tj], 0, 0, 0);
+    }
+    __syncthreads();  // every wavefront has read its X
+    // D[kk + 4r][i16] of tile tj = X_a[level kk + 4r][j = 16 tj + i16] -> xs[c][j][l]
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xs[(wv * K + 16 * tj + i16) * XM_L + kk + 4 * r] = acc[tj][r];
+    __syncthreads();
+    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, j = idx / (XM_C * XM_L);
+      if (c < ncl && l < nll) X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = xs[(c * K + j) * XM_L + l];
+    }
+  }
+}
+
 bool letkf_transform_reg_supported(int k) {
   switch (k) {
     case 8: case 16: case 20: case 32: case 40: case 48: case 64: return true;
@@ -1268,6 +1357,18 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
   if (nc < 0) nc = ncol;
   MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
   if (nc == 0) return;
+  if (k % 16 == 0 && k <= 64) {
+    MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
+    const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
+    switch (k) {
+      case 16: hipLaunchKernelGGL(k_letkf_transform_mfma<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
+      case 32: hipLaunchKernelGGL(k_letkf_transform_mfma<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
+      case 48: hipLaunchKernelGGL(k_letkf_transform_mfma<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
+      default: hipLaunchKernelGGL(k_letkf_transform_mfma<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
+    }
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   if (letkf_transform_reg_supported(k)) {
     MDA_REQUIRE((nc + XT_C - 1) / XT_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbr = (unsigned)(((nc + XT_C - 1) / XT_C + 7) / 8 * 8);

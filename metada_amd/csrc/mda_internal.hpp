@@ -266,7 +266,6 @@ struct letkf_solve_args {
   double* lam;   // [ncol][k]
   double* U;     // [ncol][k][k] Householder reflectors
   int* flag;     // [ncol] columns re-solved by the Jacobi fallback
-  double* Zs;    // [ncol][k][k] eigenvector scratch (k > 64)
   // Jacobi in global memory (k too large for LDS): per-block scratch of
   // 2 kp (kp + 1) doubles for jblocks blocks
   double* jscratch;
