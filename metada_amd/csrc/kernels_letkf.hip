@@ -498,6 +498,9 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
   // staging: BF16x3 uses hi [64][GM_LD] + lo [64][GM_LD] bf16; FP32 [64][GM_LDF]
   constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * 64 * GM_LD * 2 : 64 * GM_LDF * 4;
   __shared__ __attribute__((aligned(16))) unsigned char stg_raw[4][STG_BYTES];
+  // rows of the batch in flight: [32 observations][64 members] FP32, filled
+  // by global_load_lds (no registers held while the rows are in flight)
+  __shared__ __attribute__((aligned(16))) float rows_raw[4][32 * 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int k = a.k;
@@ -524,17 +527,37 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
       vx = cos(la) * cos(lo); vy = cos(la) * sin(lo); vz = sin(la);
     }
-    // consume the first 32 (or `cnt` < 32 at the end) queued observations
-    auto consume = [&](int cnt) {
-      // lane j < 32 holds queue entry j; entry j is broadcast to the wave by
-      // readlane (scalar registers), so the 32 row loads need no LDS reads.
-      // Entries j >= cnt read row 0 with zero scale and weight.  Rows are
-      // staged 8 at a time to keep few values live.  b takes FP32 partial
-      // sums over the 32 rows, added into the FP64 accumulator.
+    // Batches of 32 queued observations.  issue(cnt): the rows of queue
+    // entries [0, cnt) start towards the wave's LDS row buffer (one
+    // global_load_lds per row: lane m moves member m), and their scale and
+    // weight stay in registers (lane j holds entry j).  consume(): waits for
+    // the rows, then scales, splits and multiplies them.  The rows of a batch
+    // are in flight while the search goes on; entries j >= cnt carry zero
+    // scale and weight.
+    float* rows = rows_raw[wv];
+    float b_wd = 0.f, b_sw = 0.f;
+    bool inflight = false;
+    auto issue = [&](int cnt) {
       const int my_o = lane < cnt ? q_off[wv][lane] : 0;
-      const float my_wd = lane < cnt ? q_wd[wv][lane] : 0.f;
-      const float my_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
-      const float* yrow = a.y32 + (lane < k ? lane : 0);
+      b_wd = lane < cnt ? q_wd[wv][lane] : 0.f;
+      b_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
+      const float* ycol = a.y32 + (lane < k ? lane : 0);
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int o = __builtin_amdgcn_readlane(my_o, j);
+        __builtin_amdgcn_global_load_lds((const void*)(ycol + (size_t)o * k),
+                                         (__attribute__((address_space(3))) void*)(rows + 64 * j),
+                                         4, 0, 0);
+      }
+      inflight = true;
+    };
+    auto consume = [&]() {
+      // the wave's outstanding vector memory operations, the row copies
+      // among them, have completed
+      __builtin_amdgcn_s_waitcnt(0);
+      wave_sync_lds();
+      inflight = false;
+      const float my_wd = b_wd, my_sw = b_sw;
       float bp0 = 0.f, bp1 = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -542,10 +565,9 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int j = 8 * c + e;
-          const int o = __builtin_amdgcn_readlane(my_o, j);
           const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
           const float wd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_wd), j));
-          const float y = lane < k ? yrow[(size_t)o * k] : 0.f;
+          const float y = lane < k ? rows[64 * j + lane] : 0.f;
           if (e & 1) bp1 = fmaf(wd, y, bp1);
           else bp0 = fmaf(wd, y, bp0);
           av[e] = sw * y;
@@ -668,7 +690,8 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
           pend += cnt;
           wave_sync_lds();
           while (pend >= 32) {
-            consume(32);
+            if (inflight) consume();
+            issue(32);
             // move the remainder [32, pend) to the front of the queue
             const int rest = pend - 32;
             int mo = 0;
@@ -682,7 +705,11 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
         }
       }
     }
-    if (pend > 0) consume(pend);
+    if (inflight) consume();
+    if (pend > 0) {
+      issue(pend);
+      consume();
+    }
     // Projection onto the complement of the ensemble-mean direction:
     // Y'1 = 0 exactly, so C1 = 0 and 1^T b = 0 in exact arithmetic; the
     // reduced-precision products break this at their rounding level and the
