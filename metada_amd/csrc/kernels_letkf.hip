@@ -644,8 +644,24 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       for (int sg = 0; sg < cw.sl.n; ++sg) {
         const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
         const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
+        // Candidate data one batch ahead: the location (unit vector, or
+        // the GRID / CARTESIAN coordinates), 1/var and d of candidate
+        // base + 64 + lane load while batch `base` is tested.
+        const bool geo = a.coord == MDA_COORD_GEOGRAPHIC;
+        const double* pa = geo ? a.ux : a.oa;
+        const double* pb = geo ? a.uy : a.ob;
+        const double* pc = geo ? a.uz : (a.coord == MDA_COORD_CARTESIAN ? a.oc : a.ob);
+        double na = 0.0, nbv = 0.0, nc = 0.0, nr = 0.0, nd = 0.0;
+        if (s0 + lane < s1) {
+          const int64_t t0 = s0 + lane;
+          na = pa[t0]; nbv = pb[t0]; nc = pc[t0]; nr = a.rinv[t0]; nd = a.d[t0];
+        }
         for (int64_t base = s0; base < s1; base += 64) {
           const int64_t t = base + lane;
+          const double ca = na, cb = nbv, cc = nc, crinv = nr, cd = nd;
+          if (t + 64 < s1) {
+            na = pa[t + 64]; nbv = pb[t + 64]; nc = pc[t + 64]; nr = a.rinv[t + 64]; nd = a.d[t + 64];
+          }
           float w = 0.f;
           bool in = false;
           if (t < s1) {
@@ -653,16 +669,16 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
             // haversine test d <= r); the weight in FP32, the precision of
             // the contraction it feeds
             float dist;
-            if (a.coord == MDA_COORD_GEOGRAPHIC) {
-              const double dx = vx - a.ux[t], dy = vy - a.uy[t], dz = vz - a.uz[t];
+            if (geo) {
+              const double dx = vx - ca, dy = vy - cb, dz = vz - cc;
               const double c2 = dx * dx + dy * dy + dz * dz;
               in = c2 <= chord2_r;
               dist = 2.f * (float)kEarthRadiusKm * asinf(fminf(1.f, 0.5f * sqrtf((float)c2)));
             } else {
-              const double dx = cw.ca - a.oa[t], dy = cw.cb - a.ob[t];
+              const double dx = cw.ca - ca, dy = cw.cb - cb;
               double d2 = dx * dx + dy * dy;
               if (a.coord == MDA_COORD_CARTESIAN) {
-                const double dz = cw.cc - a.oc[t];
+                const double dz = cw.cc - cc;
                 d2 += dz * dz;
               }
               in = sqrt(d2) <= a.radius;
@@ -673,7 +689,7 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
             if (in)
               w = fmaxf((a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn_f32(dist / cgcf)
                                                                 : 1.f) *
-                            (float)a.rinv[t],
+                            (float)crinv,
                         0.f);
           }
           nl += __popcll(__ballot(in));
@@ -684,7 +700,7 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
           if (w != 0.f) {
             const int slot = pend + __popcll(wm & below);
             q_off[wv][slot] = (int)t;
-            q_wd[wv][slot] = w * (float)a.d[t];
+            q_wd[wv][slot] = w * (float)cd;
             q_sw[wv][slot] = sqrtf(w);
           }
           pend += cnt;
