@@ -275,7 +275,9 @@ def main():
                        PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_values"]),
         "eig_vectors": ("k_trd_vectors", "mfma", e3 / max(stages["ms_eig_vectors"], 1e-9) / 1e9,
                         PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_vectors"]),
-        "transform": ("k_letkf_transform", "hbm", xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
+        "transform": ("k_letkf_transform_mfma" if k % 16 == 0 and k <= 64 else
+                      "k_letkf_transform_reg" if k in (8, 20, 40) else "k_letkf_transform", "hbm",
+                      xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
                       PEAK["hbm_GBs"], "GB/s", stages["ms_transform"]),
     }
     dom = max(cand, key=lambda kk: cand[kk][5])
