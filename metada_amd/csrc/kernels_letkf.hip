@@ -1310,7 +1310,8 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
 constexpr int XM_C = 4, XM_L = 16;
 
 template <int K>
-__global__ __launch_bounds__(256) void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
                                                               int64_t nlev,
                                                               const double* __restrict__ T,
                                                               int64_t col0, int64_t nc) {
@@ -1350,18 +1351,35 @@ __global__ __launch_bounds__(256) void k_letkf_transform_mfma(double* __restrict
       for (int s4 = 0; s4 < NS; ++s4) tb[s4][tj] += add;
     }
   }
-  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
-    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
-    __syncthreads();  // the previous chunk's output has left xs
-    // stage X[m][l0 + l][g0 + c] -> xs[c][m][l]; consecutive threads take
-    // consecutive columns of one (member, level) row segment
-    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
+  // X[m][l0 + l][g0 + c] of one chunk, NV values per thread; consecutive
+  // threads take consecutive columns of one (member, level) row segment.
+  // The next chunk's loads are in flight during this chunk's products and
+  // output.
+  constexpr int NV = XM_C * XM_L * K / 256;
+  double pv[NV];
+  auto load_chunk = [&](int64_t l0, int nll) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 256 * q;
       const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
       double v = 0.0;
       if (c < ncl && l < nll) v = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c];
-      xs[(c * K + m) * XM_L + l] = v;
+      pv[q] = v;
+    }
+  };
+  load_chunk(0, (int)(nlev < XM_L ? nlev : XM_L));
+  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
+    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
+    __syncthreads();  // the previous chunk's output has left xs
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 256 * q;
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
+      xs[(c * K + m) * XM_L + l] = pv[q];
     }
     __syncthreads();
+    if (l0 + XM_L < nlev)
+      load_chunk(l0 + XM_L, (int)(nlev - l0 - XM_L < XM_L ? nlev - l0 - XM_L : XM_L));
     d4 acc[NTJ];
 #pragma unroll
     for (int tj = 0; tj < NTJ; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
