@@ -35,6 +35,18 @@ __device__ inline void wave_lds_sync() {
 // registers across loops, which would otherwise cost occupancy and spills.
 __device__ inline void lds_reload_point() { asm volatile("" ::: "memory"); }
 
+// Waits until at most V vector memory operations of the wave are
+// outstanding (vmcnt only; gfx9 s_waitcnt encoding: vmcnt bits 3:0 and
+// 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their no-wait maxima).  Vector
+// memory operations complete in issue order, so this covers every operation
+// issued before the last V.
+template <int V>
+__device__ inline void wait_vmcnt() {
+  static_assert(V >= 0 && V < 64, "vmcnt is six bits");
+  __builtin_amdgcn_s_waitcnt((V & 15) | ((V >> 4) << 14) | (7 << 4) | (15 << 8));
+  lds_reload_point();
+}
+
 // Pulls `bytes` at `src` into the L2 without waiting or holding registers:
 // one dword of every 128-byte line is loaded by global_load_lds into an LDS
 // sink that nothing reads.  The lines are then L2 hits when the data is used.
@@ -533,20 +545,43 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     }
     // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
     // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
-    // r = 0) are the same for every lane: they are read through the scalar
-    // cache into SGPRs and used directly as FMA operands.  Reflectors are
+    // r = 0) are the same for every lane.  They are copied into the Z area
+    // (free between the near-pair step and U_s) by global_load_lds, 256
+    // bytes per instruction and no registers held, two chunks of 16 rows
+    // ahead of their use, and read back as LDS broadcasts.  Reflectors are
     // taken in chunks of 16 so the static inner range starts at the chunk's
     // first non-zero position.
     {
       const double* Ucol = Ug + (size_t)g * n * n;
       constexpr int NC = (N - 2 + 15) / 16;
+      // rows 16c .. min(16c + 15, N - 3) to Z + 16c N; the last instruction
+      // of a chunk may read up to 252 bytes past it, inside the column's
+      // n x n block (and inside Z)
+      auto stage = [&](int c) {
+        const int jlo = 16 * c;
+        const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
+        const int bytes = (jhi - jlo + 1) * N * 8;
+        const char* src = (const char*)(Ucol + (size_t)jlo * N) + 4 * lane;
+        char* dst = (char*)(Z + jlo * N);
+        for (int off = 0; off < bytes; off += 256)
+          __builtin_amdgcn_global_load_lds((const void*)(src + off),
+                                           (__attribute__((address_space(3))) void*)(dst + off),
+                                           4, 0, 0);
+      };
+      stage(NC - 1);
+      if (NC > 1) stage(NC - 2);
 #pragma unroll
       for (int c = NC - 1; c >= 0; --c) {
         const int jlo = 16 * c;
         const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
+        // chunk c has landed once at most chunk c-1's copies (a full chunk:
+        // 16 N 8 / 256 = N / 2 instructions) are outstanding
+        if (c > 0) wait_vmcnt<N / 2>();
+        else wait_vmcnt<0>();
+        if (c >= 2) stage(c - 2);
 #pragma unroll 1
         for (int j = jhi; j >= jlo; --j) {
-          const double* u = Ucol + j * N;
+          const double* u = Z + j * N;
           const double tau = u[0];
           double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
