@@ -370,6 +370,16 @@ __device__ inline double safe_pivot(double v, double tiny) {
   return fabs(v) < tiny ? (v < 0.0 ? -tiny : tiny) : v;
 }
 
+// a / b for the recurrences of the twisted factorisation: hardware
+// reciprocal and two Newton steps (a shorter dependency chain than the IEEE
+// division sequence; about 1 ulp).  b is a safeguarded pivot: |b| >= tiny > 0.
+__device__ inline double fast_div(double a, double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  y = fma(y, fma(-b, y, 1.0), y);
+  y = fma(y, fma(-b, y, 1.0), y);
+  return a * y;
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void k_trd_vectors(
     int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
 #pragma unroll
       for (int i = 0; i < N - 1; ++i) {
         if ((i & 7) == 0) lds_reload_point();
-        z[i] = se[i] / safe_pivot(dp, tiny);  // L_i
+        z[i] = fast_div(se[i], safe_pivot(dp, tiny));  // L_i
         dp = sd[i + 1] - l - z[i] * se[i];
       }
       z[N - 1] = 0.0;
@@ -428,7 +438,7 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
 #pragma unroll
       for (int i = N - 2; i >= 0; --i) {
         if ((i & 7) == 6) lds_reload_point();
-        const double u = se[i] / safe_pivot(dm, tiny);
+        const double u = fast_div(se[i], safe_pivot(dm, tiny));
         Uv[i * 64 + lane] = u;
         dm = sd[i] - l - u * se[i];
         const double dpi = sd[i] - l - (i > 0 ? z[i > 0 ? i - 1 : 0] * se[i > 0 ? i - 1 : 0] : 0.0);
@@ -907,7 +917,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
       for (int i = 0; i < N - 1; ++i) {
         if ((i & 7) == 0) lds_reload_point();
         z[i] = safe_pivot(dp, tiny);
-        dp = sd[i + 1] - l - se[i] / z[i] * se[i];
+        dp = sd[i + 1] - l - fast_div(se[i], z[i]) * se[i];
       }
       z[N - 1] = dp;
       double gbest = fabs(dp);
@@ -916,7 +926,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
 #pragma unroll
       for (int i = N - 2; i >= 0; --i) {
         if ((i & 7) == 6) lds_reload_point();
-        const double u = se[i] / safe_pivot(dm, tiny);
+        const double u = fast_div(se[i], safe_pivot(dm, tiny));
         dm = sd[i] - l - u * se[i];
         const double gam = fabs(z[i] + dm - (sd[i] - l));
         if (gam < gbest) { gbest = gam; kt = i; }
@@ -930,7 +940,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
 #pragma unroll
       for (int i = N - 2; i >= 0; --i) {
         if ((i & 7) == 6) lds_reload_point();
-        const double u = se[i] / safe_pivot(dm, tiny);
+        const double u = fast_div(se[i], safe_pivot(dm, tiny));
         dm = sd[i] - lr - u * se[i];
         if (i >= kt) z[i + 1] = u;
       }
@@ -938,7 +948,7 @@ void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* _
 #pragma unroll
       for (int i = N - 2; i >= 0; --i) {
         if ((i & 7) == 6) lds_reload_point();
-        if (i < kt) { zc = -(se[i] / z[i]) * zc; z[i] = zc; }  // z_i = -L_i z_{i+1}
+        if (i < kt) { zc = -fast_div(se[i], z[i]) * zc; z[i] = zc; }  // z_i = -L_i z_{i+1}
       }
       double nrm = 0.0;
       zc = 1.0;
