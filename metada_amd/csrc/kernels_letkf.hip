@@ -637,6 +637,22 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       }
       wave_sync_lds();
     };
+    // every full batch of the queue: its rows go out, the batch before is
+    // consumed, the remainder moves to the front of the queue
+    auto drain = [&]() {
+      while (pend >= 32) {
+        if (inflight) consume();
+        issue(32);
+        const int rest = pend - 32;
+        int mo = 0;
+        float mwd = 0.f, msw = 0.f;
+        if (lane < rest) { mo = q_off[wv][32 + lane]; mwd = q_wd[wv][32 + lane]; msw = q_sw[wv][32 + lane]; }
+        wave_sync_lds();
+        if (lane < rest) { q_off[wv][lane] = mo; q_wd[wv][lane] = mwd; q_sw[wv][lane] = msw; }
+        wave_sync_lds();
+        pend = rest;
+      }
+    };
     // segment bounds as scalars (no dynamically indexed local arrays)
     const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
     const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
@@ -657,6 +673,10 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
           na = pa[t0]; nbv = pb[t0]; nc = pc[t0]; nr = a.rinv[t0]; nd = a.d[t0];
         }
         for (int64_t base = s0; base < s1; base += 64) {
+          // full batches queued by the previous candidates go out here, before
+          // the next candidates' loads: consume()'s wait then also covers
+          // this batch's candidate data, which is needed next anyway
+          drain();
           const int64_t t = base + lane;
           const double ca = na, cb = nbv, cc = nc, crinv = nr, cd = nd;
           if (t + 64 < s1) {
@@ -705,22 +725,10 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
           }
           pend += cnt;
           wave_sync_lds();
-          while (pend >= 32) {
-            if (inflight) consume();
-            issue(32);
-            // move the remainder [32, pend) to the front of the queue
-            const int rest = pend - 32;
-            int mo = 0;
-            float mwd = 0.f, msw = 0.f;
-            if (lane < rest) { mo = q_off[wv][32 + lane]; mwd = q_wd[wv][32 + lane]; msw = q_sw[wv][32 + lane]; }
-            wave_sync_lds();
-            if (lane < rest) { q_off[wv][lane] = mo; q_wd[wv][lane] = mwd; q_sw[wv][lane] = msw; }
-            wave_sync_lds();
-            pend = rest;
-          }
         }
       }
     }
+    drain();
     if (inflight) consume();
     if (pend > 0) {
       issue(pend);
