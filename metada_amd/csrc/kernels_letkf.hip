@@ -258,7 +258,8 @@ __device__ inline void wave_sync_lds() {
 struct col_window {
   double ca, cb, cc;
   int row0, row1;
-  seg_list sl;
+  seg_list sl;  // bucket columns of the whole window (widest row)
+  bool full;    // every longitude (GEOGRAPHIC window reaching a pole)
 };
 
 __device__ inline col_window column_window(const letkf_gram_args& a, int64_t g) {
@@ -286,7 +287,78 @@ __device__ inline col_window column_window(const letkf_gram_args& a, int64_t g) 
   w.row0 = clampi((int)floor((r_lo - G.a0) / G.da), 0, G.nrow - 1);
   w.row1 = clampi((int)floor((r_hi - G.a0) / G.da), 0, G.nrow - 1);
   w.sl = col_segments(G, c_lo, c_hi, full);
+  w.full = full;
   return w;
+}
+
+// Bucket columns of one bucket row of the window, trimmed to the circle's
+// extent inside that row's band (the window's own segments are those of its
+// widest row).  GEOGRAPHIC: the cap of angular radius d around (lat0, lon0)
+// spans longitudes lon0 +- dl(phi) at latitude phi, with
+//   cos d = sin lat0 sin phi + cos lat0 cos phi cos dl,
+// largest at sin phi_m = sin lat0 / cos d; over a band the largest extent is
+// at phi_m clamped to the band (and a band that the cap does not reach has
+// none).  GRID / CARTESIAN: the half-width sqrt(r^2 - dy^2) at the band's
+// nearest y.  Margins keep every observation that the exact distance test
+// accepts: the trimmed range only drops observations outside the circle, so
+// the in-circle observations keep their order.
+__device__ inline seg_list row_segments(const letkf_gram_args& a, const col_window& w, int row) {
+  const bucket_grid& G = a.grid;
+  seg_list none;
+  none.n = 0;
+  if (w.full) return w.sl;
+  const double band_lo = G.a0 + row * G.da, band_hi = band_lo + G.da;
+  if (a.coord == MDA_COORD_GEOGRAPHIC) {
+    const double dd = a.radius / kEarthRadiusKm;
+    const double lat0 = w.ca * kPi / 180.0;
+    const double sm = sin(lat0) / cos(dd);
+    const double phm = sm >= 1.0 ? 90.0 : sm <= -1.0 ? -90.0 : asin(sm) * 180.0 / kPi;
+    const double ph = fmin(fmax(phm, band_lo), band_hi) * kPi / 180.0;
+    const double cph = cos(ph), cl0 = cos(lat0);
+    if (cph <= 1e-9 || cl0 <= 1e-9) return w.sl;
+    const double cdl = (cos(dd) - sin(lat0) * sin(ph)) / (cl0 * cph) - 1e-12;
+    if (cdl > 1.0) return none;
+    if (cdl <= -1.0) return w.sl;
+    const double dl = acos(cdl) * 180.0 / kPi + 1e-9;
+    const double lon = norm_lon(w.cb);
+    return col_segments(G, lon - dl, lon + dl, false);
+  }
+  const double dy = w.cb < band_lo ? band_lo - w.cb : (w.cb > band_hi ? w.cb - band_hi : 0.0);
+  const double r = a.radius;
+  if (dy > r * (1.0 + 1e-12) + 1e-9) return none;
+  const double dx = sqrt(fmax(0.0, r * r - dy * dy)) * (1.0 + 1e-12) + 1e-9;
+  return col_segments(G, w.ca - dx, w.ca + dx, false);
+}
+
+// Per-row segments of a window, computed before the candidate loop into a
+// per-wave LDS table (the trigonometry's temporaries are then dead during the
+// loop).  Entry: n, j0[0], j1[0], j0[1], j1[1]; entry RT_ROWS holds the
+// window's own segments, used for rows past RT_ROWS.
+constexpr int RT_ROWS = 16;
+__device__ inline void fill_row_table(const letkf_gram_args& a, const col_window& cw,
+                                      int (*tab)[5]) {
+  const int lane = threadIdx.x & 63;
+  const int nr = cw.row1 - cw.row0 + 1;
+  if (lane <= RT_ROWS && (lane < nr || lane == RT_ROWS)) {
+    const seg_list rs = lane < RT_ROWS ? row_segments(a, cw, cw.row0 + lane) : cw.sl;
+    tab[lane][0] = rs.n;
+    tab[lane][1] = rs.j0[0];
+    tab[lane][2] = rs.j1[0];
+    tab[lane][3] = rs.n > 1 ? rs.j0[1] : 0;
+    tab[lane][4] = rs.n > 1 ? rs.j1[1] : 0;
+  }
+  wave_sync_lds();
+}
+
+__device__ inline seg_list row_from_table(const col_window& cw, int row, const int (*tab)[5]) {
+  const int i = min(row - cw.row0, RT_ROWS);
+  seg_list rs;
+  rs.n = tab[i][0];
+  rs.j0[0] = tab[i][1];
+  rs.j1[0] = tab[i][2];
+  rs.j0[1] = tab[i][3];
+  rs.j1[1] = tab[i][4];
+  return rs;
 }
 
 // Column iteration of the Gram kernels: a grid-stride walk, or (work-queue
@@ -307,13 +379,15 @@ __global__ void k_column_cost(letkf_gram_args a, uint32_t* __restrict__ keys,
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.ncol;
        g += (int64_t)gridDim.x * blockDim.x) {
     const col_window cw = column_window(a, g);
-    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
-    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
     int64_t cost = 0;
-    for (int row = cw.row0; row <= cw.row1; ++row)
-      for (int sg = 0; sg < cw.sl.n; ++sg)
+    for (int row = cw.row0; row <= cw.row1; ++row) {
+      const seg_list rs = row_segments(a, cw, row);
+      const int sj00 = rs.j0[0], sj01 = rs.j1[0];
+      const int sj10 = rs.n > 1 ? rs.j0[1] : 0, sj11 = rs.n > 1 ? rs.j1[1] : 0;
+      for (int sg = 0; sg < rs.n; ++sg)
         cost += a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1] -
                 a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
+    }
     keys[g] = 0xffffffffu - (uint32_t)(cost < 0xffffffffLL ? cost : 0xffffffffLL);
     cols[g] = (uint32_t)g;
   }
@@ -335,6 +409,7 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   constexpr int NA = (NT + NPART - 1) / NPART;
   __shared__ int q_off[4][64];
   __shared__ double q_w[4][64];
+  __shared__ int rtab[4][RT_ROWS + 1][5];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i16 = lane & 15, kk = lane >> 4;
   const int k = a.k;
@@ -353,10 +428,11 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
     for (int t = 0; t < KT; ++t) bp[t] = 0.0;
     int nl = 0;
     const col_window cw = column_window(a, g);
+    fill_row_table(a, cw, rtab[wv]);
     const double ca = cw.ca, cb = cw.cb, cc = cw.cc;
     const int row0 = cw.row0, row1 = cw.row1;
-    const seg_list sl = cw.sl;
     for (int row = row0; row <= row1; ++row) {
+      const seg_list sl = row_from_table(cw, row, rtab[wv]);
       for (int sg = 0; sg < sl.n; ++sg) {
         int64_t s0 = a.bstart[row * G.ncolb + sl.j0[sg]];
         int64_t s1 = a.bstart[row * G.ncolb + sl.j1[sg] + 1];
@@ -495,6 +571,7 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
   __shared__ int q_off[4][GM_Q];
   __shared__ float q_wd[4][GM_Q];   // w_o d_o (for b)
   __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
+  __shared__ int rtab[4][RT_ROWS + 1][5];
   // staging: BF16x3 uses hi [64][GM_LD] + lo [64][GM_LD] bf16; FP32 [64][GM_LDF]
   constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * 64 * GM_LD * 2 : 64 * GM_LDF * 4;
   __shared__ __attribute__((aligned(16))) unsigned char stg_raw[4][STG_BYTES];
@@ -516,12 +593,13 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
   int64_t it = a.queue ? queue_next(a.queue) : vb * 4 + wv;
   for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + nb * 4) {
     const int64_t g = a.order ? (int64_t)a.order[it] : it;
+    const col_window cw = column_window(a, g);
+    fill_row_table(a, cw, rtab[wv]);
     f4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     double bacc = 0.0;
     int nl = 0, pend = 0;
-    const col_window cw = column_window(a, g);
     double vx = 0.0, vy = 0.0, vz = 0.0;
     if (a.coord == MDA_COORD_GEOGRAPHIC) {
       const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
@@ -653,11 +731,12 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
         pend = rest;
       }
     };
-    // segment bounds as scalars (no dynamically indexed local arrays)
-    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
-    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
     for (int row = cw.row0; row <= cw.row1; ++row) {
-      for (int sg = 0; sg < cw.sl.n; ++sg) {
+      // this row's segments, bounds as scalars (no dynamically indexed arrays)
+      const seg_list rs = row_from_table(cw, row, rtab[wv]);
+      const int sj00 = rs.j0[0], sj01 = rs.j1[0];
+      const int sj10 = rs.n > 1 ? rs.j0[1] : 0, sj11 = rs.n > 1 ? rs.j1[1] : 0;
+      for (int sg = 0; sg < rs.n; ++sg) {
         const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
         const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
         // Candidate data one batch ahead: the location (unit vector, or
@@ -874,12 +953,14 @@ __global__ __launch_bounds__(256) void k_letkf_features(letkf_gram_args a,
   const int64_t n = a.ncol * nlev;
   for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
     const col_window cw = column_window(a, g);
-    const int sj00 = cw.sl.j0[0], sj01 = cw.sl.j1[0];
-    const int sj10 = cw.sl.n > 1 ? cw.sl.j0[1] : 0, sj11 = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
+
     double Sw = 0.0, Sd2 = 0.0, Sd = 0.0, Ssp = 0.0;
     int nl = 0;
     for (int row = cw.row0; row <= cw.row1; ++row) {
-      for (int sg = 0; sg < cw.sl.n; ++sg) {
+      const seg_list rs = row_segments(a, cw, row);
+      const int sj00 = rs.j0[0], sj01 = rs.j1[0];
+      const int sj10 = rs.n > 1 ? rs.j0[1] : 0, sj11 = rs.n > 1 ? rs.j1[1] : 0;
+      for (int sg = 0; sg < rs.n; ++sg) {
         const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
         const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
         for (int64_t t = s0 + lane; t < s1; t += 64) {
