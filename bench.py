@@ -335,8 +335,7 @@ def run_batch(a, log):
     ks = [int(x) for x in a.batch_k.split(",")]
     p = a.p if a.p != 1_000_000 else 2_000_000
     codes = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}
-    # reduced-precision contractions exist for k <= 64; wider ensembles run FP64
-    precs = [a.precision if k <= 64 else "fp64" for k in ks]
+    precs = [a.precision for k in ks]
     # the one-at-a-time runs use a context of their own, closed (workspace
     # freed) before the batch allocates its per-analysis buffers
     ctx = mda.Context(0)

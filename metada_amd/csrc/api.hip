@@ -476,8 +476,6 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   P.n = P.de.n; P.nx = ens->nx; P.ny = ens->ny; P.nlev = ens->nlev;
   const int64_t ncol = P.ncol = P.nx * P.ny;
   MDA_REQUIRE(k <= 128, "LETKF supports k <= 128 in this build");
-  MDA_REQUIRE(k <= 64 || prm->precision == MDA_PREC_FP64,
-              "FP32 / BF16x3 contractions support k <= 64 in this build");
   const bool gd = geom->on_device != 0;
   const double* ca = to_device(c, pre + "geom_a", geom->col_a, ncol, gd);
   const double* cb = to_device(c, pre + "geom_b", geom->col_b, ncol, gd);

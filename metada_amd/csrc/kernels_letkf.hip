@@ -564,20 +564,23 @@ __device__ inline double readlane_f64(double v, int l) {
 constexpr int GM_LD = 40;  // bf16 row stride of the staging tile (32 + 8 pad)
 constexpr int GM_LDF = 36; // float row stride (32 + 4 pad)
 
+// KT > 4 (64 < k <= 128): lane m handles members m and m + 64 (MB = 2), the
+// 36 lower tiles stay in one wave's registers (one wave per SIMD).
 template <int KT, int PREC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_letkf_gram_mixed(letkf_gram_args a) {
+__device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   constexpr int NT = KT * (KT + 1) / 2;
+  constexpr int MB = KT > 4 ? 2 : 1;  // members per lane
+  constexpr int KM = 64 * MB;         // member capacity
   __shared__ int q_off[4][GM_Q];
   __shared__ float q_wd[4][GM_Q];   // w_o d_o (for b)
   __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
   __shared__ int rtab[4][RT_ROWS + 1][5];
-  // staging: BF16x3 uses hi [64][GM_LD] + lo [64][GM_LD] bf16; FP32 [64][GM_LDF]
-  constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * 64 * GM_LD * 2 : 64 * GM_LDF * 4;
+  // staging: BF16x3 uses hi [KM][GM_LD] + lo [KM][GM_LD] bf16; FP32 [KM][GM_LDF]
+  constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * KM * GM_LD * 2 : KM * GM_LDF * 4;
   __shared__ __attribute__((aligned(16))) unsigned char stg_raw[4][STG_BYTES];
-  // rows of the batch in flight: [32 observations][64 members] FP32, filled
+  // rows of the batch in flight: [32 observations][KM members] FP32, filled
   // by global_load_lds (no registers held while the rows are in flight)
-  __shared__ __attribute__((aligned(16))) float rows_raw[4][32 * 64];
+  __shared__ __attribute__((aligned(16))) float rows_raw[4][32 * KM];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int k = a.k;
@@ -598,7 +601,9 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
     f4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    double bacc = 0.0;
+    double bacc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) bacc[mb] = 0.0;
     int nl = 0, pend = 0;
     double vx = 0.0, vy = 0.0, vz = 0.0;
     if (a.coord == MDA_COORD_GEOGRAPHIC) {
@@ -619,13 +624,16 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       const int my_o = lane < cnt ? q_off[wv][lane] : 0;
       b_wd = lane < cnt ? q_wd[wv][lane] : 0.f;
       b_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
-      const float* ycol = a.y32 + (lane < k ? lane : 0);
 #pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        const int o = __builtin_amdgcn_readlane(my_o, j);
-        __builtin_amdgcn_global_load_lds((const void*)(ycol + (size_t)o * k),
-                                         (__attribute__((address_space(3))) void*)(rows + 64 * j),
-                                         4, 0, 0);
+      for (int mb = 0; mb < MB; ++mb) {
+        const float* ycol = a.y32 + (64 * mb + lane < k ? 64 * mb + lane : 0);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          const int o = __builtin_amdgcn_readlane(my_o, j);
+          __builtin_amdgcn_global_load_lds(
+              (const void*)(ycol + (size_t)o * k),
+              (__attribute__((address_space(3))) void*)(rows + KM * j + 64 * mb), 4, 0, 0);
+        }
       }
       inflight = true;
     };
@@ -636,6 +644,9 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       wave_sync_lds();
       inflight = false;
       const float my_wd = b_wd, my_sw = b_sw;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+      const int m = 64 * mb + lane;  // this lane's member in block mb
       float bp0 = 0.f, bp1 = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -645,7 +656,7 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
           const int j = 8 * c + e;
           const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
           const float wd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_wd), j));
-          const float y = lane < k ? rows[64 * j + lane] : 0.f;
+          const float y = m < k ? rows[KM * j + m] : 0.f;
           if (e & 1) bp1 = fmaf(wd, y, bp1);
           else bp0 = fmaf(wd, y, bp0);
           av[e] = sw * y;
@@ -663,19 +674,20 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
             pl[q] = *(const uint32_t*)&lw;
           }
           uint16_t* hi = (uint16_t*)stg;
-          uint16_t* lo = hi + 64 * GM_LD;
-          *(uint4*)(hi + lane * GM_LD + 8 * c) = uint4{ph[0], ph[1], ph[2], ph[3]};
-          *(uint4*)(lo + lane * GM_LD + 8 * c) = uint4{pl[0], pl[1], pl[2], pl[3]};
+          uint16_t* lo = hi + KM * GM_LD;
+          *(uint4*)(hi + m * GM_LD + 8 * c) = uint4{ph[0], ph[1], ph[2], ph[3]};
+          *(uint4*)(lo + m * GM_LD + 8 * c) = uint4{pl[0], pl[1], pl[2], pl[3]};
         } else {
           float* f = (float*)stg;
-          *(float4*)(f + lane * GM_LDF + 8 * c) = float4{av[0], av[1], av[2], av[3]};
-          *(float4*)(f + lane * GM_LDF + 8 * c + 4) = float4{av[4], av[5], av[6], av[7]};
+          *(float4*)(f + m * GM_LDF + 8 * c) = float4{av[0], av[1], av[2], av[3]};
+          *(float4*)(f + m * GM_LDF + 8 * c + 4) = float4{av[4], av[5], av[6], av[7]};
         }
       }
-      bacc += (double)bp0 + (double)bp1;
+      bacc[mb] += (double)bp0 + (double)bp1;
+      }
       if (PREC == MDA_PREC_BF16X3) {
         const uint16_t* hi = (const uint16_t*)stg;
-        const uint16_t* lo = hi + 64 * GM_LD;
+        const uint16_t* lo = hi + KM * GM_LD;
         wave_sync_lds();
         bf8 fh[KT], fl[KT];
 #pragma unroll
@@ -822,7 +834,7 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
     //   row i of tile (ta, tb <= ta): the 16 lanes of one kq group, summed by
     //   xor-shuffles over i16; the mirrored upper part of row i is column i
     //   of the tiles below the diagonal, summed over kq and r.
-    double* rsum = (double*)stg;  // [64] row sums (staging is free now)
+    double* rsum = (double*)stg;  // [KM] row sums (staging is free now)
     {
       float rowp[KT][4], colp[KT];
 #pragma unroll
@@ -865,14 +877,22 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
       }
       wave_sync_lds();
     }
-    double tsum = lane < k ? rsum[lane] : 0.0;
+    double tsum = 0.0, bsum = 0.0;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      if (64 * mb + lane < k) {
+        tsum += rsum[64 * mb + lane];
+        bsum += bacc[mb];
+      }
+    }
     for (int off = 32; off > 0; off >>= 1) tsum += __shfl_xor(tsum, off);
     const double rk = 1.0 / (double)k;
     const double tot = tsum * rk * rk;
-    double bsum = lane < k ? bacc : 0.0;
     for (int off = 32; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
-    // b: lane m holds b_m
-    if (lane < k) a.b[(size_t)g * k + lane] = bacc - bsum * rk;
+    // b: lane m holds b_m (and b_(m+64))
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+      if (64 * mb + lane < k) a.b[(size_t)g * k + 64 * mb + lane] = bacc[mb] - bsum * rk;
     double* Cg = a.C + (size_t)g * k * k;
     int tile = 0;
 #pragma unroll
@@ -894,6 +914,18 @@ void k_letkf_gram_mixed(letkf_gram_args a) {
     wave_sync_lds();  // rsum reads done before the next column's staging
     if (lane == 0) a.nloc[g] = nl;
   }
+}
+
+template <int KT, int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_letkf_gram_mixed(letkf_gram_args a) {
+  gram_mixed_body<KT, PREC>(a);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+void k_letkf_gram_mixed_w(letkf_gram_args a) {
+  gram_mixed_body<8, PREC>(a);
 }
 
 __global__ void k_mixed_obs(int64_t p, int k, const double* __restrict__ s_yp,
@@ -1066,9 +1098,10 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
       case 2: MDA_GM(2); break;
       case 3: MDA_GM(3); break;
       case 4: MDA_GM(4); break;
-      default:
-        throw mda_error(MDA_E_UNSUPPORTED,
-                        "FP32 / BF16x3 Gram supports k <= 64 in this build (use FP64 for k > 64)");
+      default:  // 64 < k <= 128
+        if (bf) hipLaunchKernelGGL((k_letkf_gram_mixed_w<MDA_PREC_BF16X3>), dim3(nb), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_letkf_gram_mixed_w<MDA_PREC_FP32>), dim3(nb), dim3(256), 0, s, a);
+        break;
     }
 #undef MDA_GM
     MDA_LAUNCH_CHECK();

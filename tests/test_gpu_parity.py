@@ -409,3 +409,28 @@ def test_letkf_k128_matches_oracle(mda, ctx, k128_case, method, monkeypatch):
     if method == "auto":
         assert diag["n_fallback_columns"] == 0
         assert diag["ms_eig_values"] > 0
+
+
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-6), ("bf16x3", 1e-4)])
+def test_letkf_k128_mixed_precision(mda, ctx, k128_case, prec, bound, monkeypatch):
+    """k = 128 with the FP32 / BF16x3 Gram (two members per lane, one wave
+    per column): relative RMS against the oracle's FP64 analysis within
+    `bound`, same local counts."""
+    case, Xo, nloc = k128_case
+    monkeypatch.setenv("MDA_LETKF_CHUNK_COLS", "300")
+    code = {"fp32": mda._lib.PREC_FP32, "bf16x3": mda._lib.PREC_BF16X3}[prec]
+    ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                          coord=mda._lib.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+    f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                   "localization_radius": 2500.0}),
+                  geometry=geom, ctx=ctx, precision=code)
+    diag = f.Analyse()
+    monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+    assert diag["total_local_obs"] == nloc.sum()
+    Xm = ens.data
+    rms = np.sqrt(np.mean((Xm - Xo) ** 2)) / np.sqrt(np.mean(Xo ** 2))
+    assert rms < bound, rms
+    inc = np.sqrt(np.mean((Xo - case["X"]) ** 2))
+    assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 10 * bound * inc
