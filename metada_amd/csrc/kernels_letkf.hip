@@ -330,35 +330,64 @@ __device__ inline seg_list row_segments(const letkf_gram_args& a, const col_wind
   return col_segments(G, w.ca - dx, w.ca + dx, false);
 }
 
-// Per-row segments of a window, computed before the candidate loop into a
-// per-wave LDS table (the trigonometry's temporaries are then dead during the
-// loop).  Entry: n, j0[0], j1[0], j0[1], j1[1]; entry RT_ROWS holds the
-// window's own segments, used for rows past RT_ROWS.
+// Per-row candidate ranges of a window, computed before the candidate loop
+// into a per-wave LDS table, one lane per row (the trigonometry's
+// temporaries are dead during the loop, and the bucket-start loads of all
+// rows are in flight together instead of one row at a time).  Entry i < RT_ROWS
+// (row row0 + i): n, then [s0, s1) observation ranges of up to two
+// segments.  Entry RT_ROWS holds the window's own bucket columns (n, j0[0],
+// j1[0], j0[1], j1[1]) for rows past RT_ROWS, whose ranges are read in the
+// loop.
 constexpr int RT_ROWS = 16;
+struct row_ranges {
+  int n;
+  int s0[2], s1[2];
+};
+
 __device__ inline void fill_row_table(const letkf_gram_args& a, const col_window& cw,
                                       int (*tab)[5]) {
   const int lane = threadIdx.x & 63;
   const int nr = cw.row1 - cw.row0 + 1;
-  if (lane <= RT_ROWS && (lane < nr || lane == RT_ROWS)) {
-    const seg_list rs = lane < RT_ROWS ? row_segments(a, cw, cw.row0 + lane) : cw.sl;
+  const bucket_grid& G = a.grid;
+  if (lane < RT_ROWS && lane < nr) {
+    const int row = cw.row0 + lane;
+    const seg_list rs = row_segments(a, cw, row);
     tab[lane][0] = rs.n;
-    tab[lane][1] = rs.j0[0];
-    tab[lane][2] = rs.j1[0];
-    tab[lane][3] = rs.n > 1 ? rs.j0[1] : 0;
-    tab[lane][4] = rs.n > 1 ? rs.j1[1] : 0;
+    tab[lane][1] = rs.n > 0 ? a.bstart[row * G.ncolb + rs.j0[0]] : 0;
+    tab[lane][2] = rs.n > 0 ? a.bstart[row * G.ncolb + rs.j1[0] + 1] : 0;
+    tab[lane][3] = rs.n > 1 ? a.bstart[row * G.ncolb + rs.j0[1]] : 0;
+    tab[lane][4] = rs.n > 1 ? a.bstart[row * G.ncolb + rs.j1[1] + 1] : 0;
+  }
+  if (lane == RT_ROWS) {
+    tab[lane][0] = cw.sl.n;
+    tab[lane][1] = cw.sl.j0[0];
+    tab[lane][2] = cw.sl.j1[0];
+    tab[lane][3] = cw.sl.n > 1 ? cw.sl.j0[1] : 0;
+    tab[lane][4] = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
   }
   wave_sync_lds();
 }
 
-__device__ inline seg_list row_from_table(const col_window& cw, int row, const int (*tab)[5]) {
-  const int i = min(row - cw.row0, RT_ROWS);
-  seg_list rs;
-  rs.n = tab[i][0];
-  rs.j0[0] = tab[i][1];
-  rs.j1[0] = tab[i][2];
-  rs.j0[1] = tab[i][3];
-  rs.j1[1] = tab[i][4];
-  return rs;
+__device__ inline row_ranges row_from_table(const letkf_gram_args& a, const col_window& cw,
+                                            int row, const int (*tab)[5]) {
+  const int i = row - cw.row0;
+  row_ranges rr;
+  if (i < RT_ROWS) {
+    rr.n = tab[i][0];
+    rr.s0[0] = tab[i][1];
+    rr.s1[0] = tab[i][2];
+    rr.s0[1] = tab[i][3];
+    rr.s1[1] = tab[i][4];
+  } else {
+    const int* e = tab[RT_ROWS];
+    const int base = row * a.grid.ncolb;
+    rr.n = e[0];
+    rr.s0[0] = a.bstart[base + e[1]];
+    rr.s1[0] = a.bstart[base + e[2] + 1];
+    rr.s0[1] = rr.n > 1 ? a.bstart[base + e[3]] : 0;
+    rr.s1[1] = rr.n > 1 ? a.bstart[base + e[4] + 1] : 0;
+  }
+  return rr;
 }
 
 // Column iteration of the Gram kernels: a grid-stride walk, or (work-queue
@@ -432,10 +461,10 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
     const double ca = cw.ca, cb = cw.cb, cc = cw.cc;
     const int row0 = cw.row0, row1 = cw.row1;
     for (int row = row0; row <= row1; ++row) {
-      const seg_list sl = row_from_table(cw, row, rtab[wv]);
-      for (int sg = 0; sg < sl.n; ++sg) {
-        int64_t s0 = a.bstart[row * G.ncolb + sl.j0[sg]];
-        int64_t s1 = a.bstart[row * G.ncolb + sl.j1[sg] + 1];
+      const row_ranges rr = row_from_table(a, cw, row, rtab[wv]);
+      for (int sg = 0; sg < rr.n; ++sg) {
+        const int64_t s0 = sg ? rr.s0[1] : rr.s0[0];
+        const int64_t s1 = sg ? rr.s1[1] : rr.s1[0];
         for (int64_t base = s0; base < s1; base += 64) {
           int64_t t = base + lane;
           double w = 0.0;
@@ -744,13 +773,11 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
       }
     };
     for (int row = cw.row0; row <= cw.row1; ++row) {
-      // this row's segments, bounds as scalars (no dynamically indexed arrays)
-      const seg_list rs = row_from_table(cw, row, rtab[wv]);
-      const int sj00 = rs.j0[0], sj01 = rs.j1[0];
-      const int sj10 = rs.n > 1 ? rs.j0[1] : 0, sj11 = rs.n > 1 ? rs.j1[1] : 0;
-      for (int sg = 0; sg < rs.n; ++sg) {
-        const int64_t s0 = a.bstart[row * G.ncolb + (sg ? sj10 : sj00)];
-        const int64_t s1 = a.bstart[row * G.ncolb + (sg ? sj11 : sj01) + 1];
+      // this row's observation ranges (no dynamically indexed arrays)
+      const row_ranges rr = row_from_table(a, cw, row, rtab[wv]);
+      for (int sg = 0; sg < rr.n; ++sg) {
+        const int64_t s0 = sg ? rr.s0[1] : rr.s0[0];
+        const int64_t s1 = sg ? rr.s1[1] : rr.s1[0];
         // Candidate data one batch ahead: the location (unit vector, or
         // the GRID / CARTESIAN coordinates), 1/var and d of candidate
         // base + 64 + lane load while batch `base` is tested.
