@@ -484,8 +484,10 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
               "GEOGRAPHIC / CARTESIAN geometry needs column coordinates");
   const double* rho_col =
       to_device(c, pre + "rho_col", prm->rho_column, ncol, prm->rho_on_device != 0);
-  P.yp = c->get<double>(pre + "yp_om", (size_t)p * k);
-  P.dd = c->get<double>(pre + "d", p);
+  if (P.h.hx) {  // input-order Y' and d before they move into bucket order
+    P.yp = c->get<double>(pre + "yp_om", (size_t)p * k);
+    P.dd = c->get<double>(pre + "d", p);
+  }
 
   // bucket grid over the localisation coordinates
   bucket_grid& bg = P.bg;
@@ -603,27 +605,37 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   return P;
 }
 
-// Stages before the per-column work (LETKF.hpp:152-211): HX, Y', d, the
-// bucket index and the gathered observations.  Idempotent.
-static void enqueue_local(hipStream_t s, letkf_plan& P) {
-  const mda_letkf_params* prm = P.prm;
-  launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, P.h.hx, P.o.yo,
-                   nullptr, P.yp, P.dd, nullptr);
-}
-
+// Stages before the per-column work (LETKF.hpp:152-211).  The bucket index
+// comes first: the observation statistics then run in bucket order, writing
+// Y', d (and the FP32 Y') straight into the sorted arrays the Gram reads,
+// with the state gathers of neighbouring observations sharing cache lines.
+// A caller-supplied member-major H(x) is read in input order (coalesced) and
+// its rows are moved into bucket order afterwards.  Both idempotent.
 static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
   letkf_gram_args& ga = P.ga;
   launch_bucket_keys(s, P.rcoord, P.ccoord, P.p, P.bg, P.keys, P.vals);
   radix_sort_pairs(s, P.keys, P.vals, P.keys2, P.vals2, P.p, P.key_bits, P.hist, P.hist_cap);
   launch_bucket_ranges(s, P.keys, P.p, P.nbuckets, P.bstart);
-  launch_gather_obs(s, P.vals, P.p, P.k, P.yp, P.dd, P.o.var, P.o.a, P.o.b, P.o.c,
-                    P.prm->rweight, const_cast<double*>(ga.yp), const_cast<double*>(ga.d),
+  launch_gather_obs(s, P.vals, P.p, P.o.var, P.o.a, P.o.b, P.o.c, P.prm->rweight,
                     const_cast<double*>(ga.rinv), const_cast<double*>(ga.oa),
-                    const_cast<double*>(ga.ob), const_cast<double*>(ga.oc));
-  if (P.prm->precision != MDA_PREC_FP64)
-    launch_mixed_obs(s, P.p, P.k, ga.yp, ga.oa, ga.ob, P.geo, const_cast<float*>(ga.y32),
-                     const_cast<double*>(ga.ux), const_cast<double*>(ga.uy),
-                     const_cast<double*>(ga.uz));
+                    const_cast<double*>(ga.ob), const_cast<double*>(ga.oc),
+                    const_cast<double*>(ga.ux), const_cast<double*>(ga.uy),
+                    const_cast<double*>(ga.uz));
+}
+
+static void enqueue_local(hipStream_t s, letkf_plan& P) {
+  letkf_gram_args& ga = P.ga;
+  float* y32 = P.prm->precision != MDA_PREC_FP64 ? const_cast<float*>(ga.y32) : nullptr;
+  if (P.h.hx) {
+    launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, P.h.hx, P.o.yo,
+                     nullptr, P.yp, P.dd, nullptr);
+    launch_gather_rows(s, P.vals, P.p, P.k, P.yp, P.dd, const_cast<double*>(ga.yp),
+                       const_cast<double*>(ga.d), y32);
+  } else {
+    launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, nullptr, P.o.yo,
+                     nullptr, const_cast<double*>(ga.yp), const_cast<double*>(ga.d), nullptr,
+                     P.vals, y32);
+  }
 }
 
 // The plan's per-column arguments restricted to the pass of columns
@@ -746,17 +758,17 @@ int mda_letkf_analyse(mda_ctx* c, const mda_letkf_params* prm, mda_ensemble* ens
     for (int64_t i = 0; i < npass; ++i)
       for (int j = 0; j < 7; ++j) marks[i].e[j] = c->evpool[7 * i + j];
     MDA_HIP(hipEventRecord(c->ev[1], s));
-    enqueue_local(s, P);
-    MDA_HIP(hipEventRecord(c->ev[2], s));
     enqueue_buckets(s, P);
+    MDA_HIP(hipEventRecord(c->ev[2], s));
+    enqueue_local(s, P);
     MDA_HIP(hipEventRecord(c->ev[3], s));
     enqueue_columns(s, P, {0, 0}, &marks);
     MDA_HIP(hipEventRecord(c->ev[6], s));
     enqueue_stats(s, P);
     finish_letkf(c, P, diag);
     if (diag) {
-      diag->ms_obs = elapsed(c->ev[1], c->ev[2]);
-      diag->ms_bucket = elapsed(c->ev[2], c->ev[3]);
+      diag->ms_bucket = elapsed(c->ev[1], c->ev[2]);
+      diag->ms_obs = elapsed(c->ev[2], c->ev[3]);
       diag->ms_total = elapsed(c->ev[0], c->ev[6]);
       double g = 0, so = 0, tr = 0, e1 = 0, e2 = 0, e3 = 0, jf = 0;
       for (const pass_marks& m : marks) {
@@ -797,8 +809,8 @@ static std::string tune_key(const letkf_plan& P) {
 
 static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_queue) {
   hipStream_t s = c->stream;
-  enqueue_local(s, P);
   enqueue_buckets(s, P);
+  enqueue_local(s, P);
   const int cand_blocks[] = {0, 512, 1024, 2048};
   std::pair<int, int> best{0, 0};
   float best_ms = 1e30f;
@@ -820,8 +832,8 @@ static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_q
 }
 
 static void enqueue_plan(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg) {
-  enqueue_local(s, P);
   enqueue_buckets(s, P);
+  enqueue_local(s, P);
   enqueue_columns(s, P, cfg, nullptr);
   enqueue_stats(s, P);
 }
@@ -1038,8 +1050,8 @@ int mda_letkf_inflation_features(mda_ctx* c, const mda_letkf_params* prm,
     fp.rho_column = nullptr;
     letkf_plan P = plan_letkf(c, &fp, const_cast<mda_ensemble*>(ens), geom, obs, hop, "",
                               false);
-    enqueue_local(s, P);
     enqueue_buckets(s, P);
+    enqueue_local(s, P);
     const int64_t ncol = P.ncol;
     const dev_obs& o = P.o;
     const dev_ensemble& de = P.de;

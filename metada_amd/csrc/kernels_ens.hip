@@ -94,58 +94,98 @@ void launch_dev_sumsq(hipStream_t s, const double* X, int k, int64_t n, double s
 // Observation space (EnKF.hpp:168-184, LETKF.hpp:197-211): hx_j = H x_j for
 // every member (CSR stencil gather, or a caller-supplied member-major H(x)),
 // ybar = mean over members, Y' = hx - ybar, d = y_o - ybar.  A block owns 64
-// observations; the [64][k] tile is assembled in LDS so that the obs-major Y'
-// rows go out as contiguous, coalesced stores.
+// output positions; the [64][k] tile is assembled in LDS so that the
+// obs-major Y' rows go out as contiguous, coalesced stores.  With `order`,
+// output position t holds observation order[t]: in the bucket order of the
+// LETKF, the stencils of a block's observations lie close together, so the
+// state gathers of one member share cache lines.
 // ---------------------------------------------------------------------------
 constexpr int OBS_TILE = 64;
+constexpr int OBS_TAPS = 8;  // stencils held in registers (trilinear: 8)
 
 __global__ __launch_bounds__(256) void k_obs_stats(
     const double* __restrict__ X, int k, int64_t n, int64_t p,
     const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col,
     const double* __restrict__ w, const double* __restrict__ hx_mm,
-    const double* __restrict__ yo, double* __restrict__ ybar, double* __restrict__ yp_om,
-    double* __restrict__ d, double* __restrict__ yp_mm) {
-  extern __shared__ double tile[];  // [OBS_TILE][k]
+    const double* __restrict__ yo, const uint32_t* __restrict__ order,
+    double* __restrict__ ybar, double* __restrict__ yp_om, double* __restrict__ d,
+    double* __restrict__ yp_mm, float* __restrict__ y32) {
+  extern __shared__ double tile[];  // [OBS_TILE][k + 1] (padded rows)
+  const int kp = k + 1;
   __shared__ double mean_s[OBS_TILE];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t o0 = (int64_t)blockIdx.x * OBS_TILE;
   const int nob = (int)((p - o0) < OBS_TILE ? (p - o0) : OBS_TILE);
-  // H x_j: entry (ob, j) with ob fastest so hx_mm reads are coalesced
-  for (int e = tid; e < nob * k; e += nt) {
-    int ob = e % nob, j = e / nob;
-    int64_t o = o0 + ob;
-    double v;
-    if (hx_mm) {
-      v = hx_mm[(size_t)j * p + o];
-    } else {
-      v = 0.0;
-      const double* xj = X + (size_t)j * n;
-      for (int64_t t = row_ptr[o]; t < row_ptr[o + 1]; ++t) v += w[t] * xj[col[t]];
+  if (hx_mm) {
+    // H x_j: entry (ob, j) with ob fastest so hx_mm reads are coalesced
+    for (int e = tid; e < nob * k; e += nt) {
+      int ob = e % nob, j = e / nob;
+      const int64_t o = order ? (int64_t)order[o0 + ob] : o0 + ob;
+      tile[ob * kp + j] = hx_mm[(size_t)j * p + o];
     }
-    tile[ob * k + j] = v;
+  } else {
+    // Stencil: lane ob of every wave owns observation ob, wave g members
+    // g, g + 4, ...  A stencil of up to OBS_TAPS taps is read once into
+    // registers and its gathers for one member go out together; the sum
+    // runs over the taps in stencil order.
+    const int ob = tid & 63, jg = tid >> 6;
+    if (ob < nob) {
+      const int64_t o = order ? (int64_t)order[o0 + ob] : o0 + ob;
+      const int64_t t0 = row_ptr[o], t1 = row_ptr[o + 1];
+      if (t1 - t0 <= OBS_TAPS) {
+        const int nt_o = (int)(t1 - t0);
+        int64_t cc[OBS_TAPS];
+        double ww[OBS_TAPS];
+#pragma unroll
+        for (int q = 0; q < OBS_TAPS; ++q) {
+          cc[q] = q < nt_o ? col[t0 + q] : 0;  // absent taps read entry 0, unused
+          ww[q] = q < nt_o ? w[t0 + q] : 0.0;
+        }
+#pragma unroll 2
+        for (int j = jg; j < k; j += nt / 64) {
+          const double* xj = X + (size_t)j * n;
+          double xv[OBS_TAPS];
+#pragma unroll
+          for (int q = 0; q < OBS_TAPS; ++q) xv[q] = xj[cc[q]];
+          double v = 0.0;
+#pragma unroll
+          for (int q = 0; q < OBS_TAPS; ++q)
+            if (q < nt_o) v += ww[q] * xv[q];
+          tile[ob * kp + j] = v;
+        }
+      } else {
+        for (int j = jg; j < k; j += nt / 64) {
+          const double* xj = X + (size_t)j * n;
+          double v = 0.0;
+          for (int64_t t = t0; t < t1; ++t) v += w[t] * xj[col[t]];
+          tile[ob * kp + j] = v;
+        }
+      }
+    }
   }
   __syncthreads();
   for (int ob = tid; ob < nob; ob += nt) {
     double s = 0.0;
-    for (int j = 0; j < k; ++j) s += tile[ob * k + j];
+    for (int j = 0; j < k; ++j) s += tile[ob * kp + j];
     double m = s / (double)k;
     mean_s[ob] = m;
-    int64_t o = o0 + ob;
-    if (ybar) ybar[o] = m;
-    if (d) d[o] = yo[o] - m;
+    const int64_t o = order ? (int64_t)order[o0 + ob] : o0 + ob;
+    if (ybar) ybar[o0 + ob] = m;
+    if (d) d[o0 + ob] = yo[o] - m;
   }
   __syncthreads();
   for (int e = tid; e < nob * k; e += nt) {
     int ob = e / k, j = e % k;
-    double v = tile[e] - mean_s[ob];
+    double v = tile[ob * kp + j] - mean_s[ob];
     if (yp_om) yp_om[(size_t)o0 * k + e] = v;
-    tile[e] = v;
+    if (y32) y32[(size_t)o0 * k + e] = (float)v;
+    tile[ob * kp + j] = v;
   }
   if (yp_mm) {
     __syncthreads();
     for (int e = tid; e < nob * k; e += nt) {
       int ob = e % nob, j = e / nob;
-      yp_mm[(size_t)j * p + o0 + ob] = tile[ob * k + j];
+      yp_mm[(size_t)j * p + o0 + ob] = tile[ob * kp + j];
     }
   }
 }
@@ -153,12 +193,12 @@ __global__ __launch_bounds__(256) void k_obs_stats(
 void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t p,
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
-                      double* d, double* yp_mm) {
+                      double* d, double* yp_mm, const uint32_t* order, float* y32) {
   if (p <= 0) return;
   unsigned nb = (unsigned)((p + OBS_TILE - 1) / OBS_TILE);
-  size_t shm = sizeof(double) * OBS_TILE * k;
+  size_t shm = sizeof(double) * OBS_TILE * (k + 1);
   hipLaunchKernelGGL(k_obs_stats, dim3(nb), dim3(256), shm, s, X, k, n, p, row_ptr, col, w,
-                     hx_mm, yo, ybar, yp_om, d, yp_mm);
+                     hx_mm, yo, order, ybar, yp_om, d, yp_mm, y32);
   MDA_LAUNCH_CHECK();
 }
 

@@ -49,10 +49,14 @@ void launch_bucket_keys(hipStream_t s, const double* rcoord, const double* ccoor
   MDA_LAUNCH_CHECK();
 }
 
-// Stable LSD radix sort of (key, value) pairs, 8-bit digits.  Per pass: block
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits.  A block owns
+// RS_TILE consecutive keys (RS_KPT sub-tiles of RS_BLOCK).  Per pass: block
 // digit histograms (digit-major), one exclusive scan, stable scatter with
-// in-wave ranks from ballots.  Deterministic.
+// in-wave ranks from ballots and per-digit running counts across the
+// block's sub-tiles.  Deterministic.
 constexpr int RS_BLOCK = 256;
+constexpr int RS_KPT = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_KPT;
 
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict__ keys,
                                                       int64_t p, int shift,
@@ -61,33 +65,49 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  int64_t i = blockIdx.x * (int64_t)RS_BLOCK + threadIdx.x;
-  if (i < p) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  const int64_t base = blockIdx.x * (int64_t)RS_TILE + threadIdx.x;
+#pragma unroll
+  for (int st = 0; st < RS_KPT; ++st) {
+    const int64_t i = base + (int64_t)st * RS_BLOCK;
+    if (i < p) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  }
   __syncthreads();
   hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // exclusive scan of `total` uint32 in place, one workgroup of 1024 threads
+// walking the array in coalesced tiles of 4096 (4 consecutive values per
+// thread), carrying the running total between tiles
 __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ h, int64_t total) {
   __shared__ uint32_t part[1024];
+  __shared__ uint32_t carry_s;
   const int tid = threadIdx.x;
-  int64_t chunk = (total + 1023) / 1024;
-  int64_t lo = tid * chunk, hi = lo + chunk < total ? lo + chunk : total;
-  uint32_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += h[i];
-  part[tid] = s;
+  if (tid == 0) carry_s = 0;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = tid >= off ? part[tid - off] : 0u;
+  for (int64_t t0 = 0; t0 < total; t0 += 4096) {
+    const int64_t i0 = t0 + 4 * (int64_t)tid;
+    uint32_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = i0 + q < total ? h[i0 + q] : 0u;
+    const uint32_t s = v[0] + v[1] + v[2] + v[3];
+    part[tid] = s;
     __syncthreads();
-    part[tid] += v;
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t u = tid >= off ? part[tid - off] : 0u;
+      __syncthreads();
+      part[tid] += u;
+      __syncthreads();
+    }
+    const uint32_t carry = carry_s;
+    uint32_t run = carry + part[tid] - s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i0 + q < total) h[i0 + q] = run;
+      run += v[q];
+    }
     __syncthreads();
-  }
-  uint32_t run = part[tid] - s;  // exclusive prefix of this chunk
-  for (int64_t i = lo; i < hi; ++i) {
-    uint32_t v = h[i];
-    h[i] = run;
-    run += v;
+    if (tid == 1023) carry_s = carry + part[1023];
+    __syncthreads();
   }
 }
 
@@ -96,28 +116,42 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t p, int shift,
     const uint32_t* __restrict__ hist, int64_t nblocks) {
   __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+  __shared__ uint32_t run[256];  // keys of each digit in the block's earlier sub-tiles
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int t = tid; t < (RS_BLOCK / 64) * 256; t += RS_BLOCK) (&wcnt[0][0])[t] = 0;
+  run[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
   __syncthreads();
-  int64_t i = blockIdx.x * (int64_t)RS_BLOCK + tid;
-  bool valid = i < p;
-  uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
-  uint32_t dig = (key >> shift) & 255u;
-  uint64_t match = __ballot(valid);
-  for (int b = 0; b < 8; ++b) {
-    uint64_t m = __ballot(valid && ((dig >> b) & 1u));
-    match &= ((dig >> b) & 1u) ? m : ~m;
-  }
-  uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  uint32_t rank = (uint32_t)__popcll(match & below);
-  if (valid && rank == 0) wcnt[wv][dig] = (uint32_t)__popcll(match);
-  __syncthreads();
-  if (valid) {
-    uint32_t off = 0;
-    for (int w = 0; w < wv; ++w) off += wcnt[w][dig];
-    uint32_t dst = hist[(size_t)dig * nblocks + blockIdx.x] + off + rank;
-    kout[dst] = key;
-    vout[dst] = val;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int st = 0; st < RS_KPT; ++st) {
+    const int64_t i = blockIdx.x * (int64_t)RS_TILE + (int64_t)st * RS_BLOCK + tid;
+    if (blockIdx.x * (int64_t)RS_TILE + (int64_t)st * RS_BLOCK >= p) break;  // block-uniform
+    const bool valid = i < p;
+    const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+    const uint32_t dig = (key >> shift) & 255u;
+    uint64_t match = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot(valid && ((dig >> b) & 1u));
+      match &= ((dig >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(match & below);
+    if (valid && rank == 0) wcnt[wv][dig] = (uint32_t)__popcll(match);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = run[dig];
+      for (int w = 0; w < wv; ++w) off += wcnt[w][dig];
+      const uint32_t dst = off + rank;
+      kout[dst] = key;
+      vout[dst] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < RS_BLOCK / 64; ++w) {
+      add += wcnt[w][tid];
+      wcnt[w][tid] = 0;
+    }
+    run[tid] += add;
+    __syncthreads();
   }
 }
 
@@ -125,7 +159,7 @@ void radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* k
                       uint32_t* vals_tmp, int64_t p, int key_bits, uint32_t* hist,
                       int64_t hist_cap) {
   if (p <= 0) return;
-  int64_t nblocks = (p + RS_BLOCK - 1) / RS_BLOCK;
+  int64_t nblocks = (p + RS_TILE - 1) / RS_TILE;
   MDA_REQUIRE(nblocks * 256 <= hist_cap, "radix histogram workspace too small");
   uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
   int passes = 0;
@@ -166,38 +200,66 @@ void launch_bucket_ranges(hipStream_t s, const uint32_t* sorted_keys, int64_t p,
   MDA_LAUNCH_CHECK();
 }
 
-__global__ void k_gather_obs(const uint32_t* __restrict__ order, int64_t p, int k,
-                             const double* __restrict__ yp_om, const double* __restrict__ d,
+// Per-observation data in bucket order: 1/var (or 1), the coordinates, and
+// for the FP32 / BF16x3 contractions of GEOGRAPHIC problems the unit vector
+// of each location (ux, uy, uz may be null).
+__global__ void k_gather_obs(const uint32_t* __restrict__ order, int64_t p,
                              const double* __restrict__ var, const double* __restrict__ oa,
                              const double* __restrict__ ob, const double* __restrict__ oc,
-                             int rweight, double* __restrict__ s_yp, double* __restrict__ s_d,
-                             double* __restrict__ s_rinv, double* __restrict__ s_a,
-                             double* __restrict__ s_b, double* __restrict__ s_c) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < p * k;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t t = e / k;
-    int j = (int)(e % k);
-    uint32_t o = order[t];
-    s_yp[e] = yp_om[(size_t)o * k + j];
-    if (j == 0) {
-      s_d[t] = d[o];
-      s_rinv[t] = rweight == MDA_RW_IDENTITY ? 1.0 : 1.0 / var[o];
-      s_a[t] = oa[o];
-      s_b[t] = ob[o];
-      s_c[t] = oc ? oc[o] : 0.0;
+                             int rweight, double* __restrict__ s_rinv, double* __restrict__ s_a,
+                             double* __restrict__ s_b, double* __restrict__ s_c,
+                             double* __restrict__ ux, double* __restrict__ uy,
+                             double* __restrict__ uz) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < p;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = order[t];
+    const double a = oa[o], b = ob[o];
+    s_rinv[t] = rweight == MDA_RW_IDENTITY ? 1.0 : 1.0 / var[o];
+    s_a[t] = a;
+    s_b[t] = b;
+    s_c[t] = oc ? oc[o] : 0.0;
+    if (ux) {
+      const double la = a * kPi / 180.0, lo = b * kPi / 180.0;
+      ux[t] = cos(la) * cos(lo); uy[t] = cos(la) * sin(lo); uz[t] = sin(la);
     }
   }
 }
 
-void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, int k,
-                       const double* yp_om, const double* d, const double* var,
+void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, const double* var,
                        const double* oa, const double* ob, const double* oc, int rweight,
-                       double* s_yp, double* s_d, double* s_rinv, double* s_a, double* s_b,
-                       double* s_c) {
+                       double* s_rinv, double* s_a, double* s_b, double* s_c, double* ux,
+                       double* uy, double* uz) {
   if (p <= 0) return;
-  hipLaunchKernelGGL(k_gather_obs, dim3(grid_for(p * k, 256, 65536)), dim3(256), 0, s, order,
-                     p, k, yp_om, d, var, oa, ob, oc, rweight, s_yp, s_d, s_rinv, s_a, s_b,
-                     s_c);
+  hipLaunchKernelGGL(k_gather_obs, dim3(grid_for(p, 256, 65536)), dim3(256), 0, s, order, p,
+                     var, oa, ob, oc, rweight, s_rinv, s_a, s_b, s_c, ux, uy, uz);
+  MDA_LAUNCH_CHECK();
+}
+
+// Y' rows and d moved into bucket order (the path where the caller supplies
+// H(x) member-major and the observation statistics run in input order), with
+// the FP32 copy of Y' for the reduced-precision contractions (y32 may be null).
+__global__ void k_gather_rows(const uint32_t* __restrict__ order, int64_t p, int k,
+                              const double* __restrict__ yp_om, const double* __restrict__ d,
+                              double* __restrict__ s_yp, double* __restrict__ s_d,
+                              float* __restrict__ y32) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < p * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e / k;
+    const int j = (int)(e % k);
+    const uint32_t o = order[t];
+    const double v = yp_om[(size_t)o * k + j];
+    s_yp[e] = v;
+    if (y32) y32[e] = (float)v;
+    if (j == 0) s_d[t] = d[o];
+  }
+}
+
+void launch_gather_rows(hipStream_t s, const uint32_t* order, int64_t p, int k,
+                        const double* yp_om, const double* d, double* s_yp, double* s_d,
+                        float* y32) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(p * k, 256, 65536)), dim3(256), 0, s, order,
+                     p, k, yp_om, d, s_yp, s_d, y32);
   MDA_LAUNCH_CHECK();
 }
 
@@ -953,30 +1015,6 @@ template <int PREC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
 void k_letkf_gram_mixed_w(letkf_gram_args a) {
   gram_mixed_body<8, PREC>(a);
-}
-
-__global__ void k_mixed_obs(int64_t p, int k, const double* __restrict__ s_yp,
-                            const double* __restrict__ s_a, const double* __restrict__ s_b,
-                            int geographic, float* __restrict__ y32, double* __restrict__ ux,
-                            double* __restrict__ uy, double* __restrict__ uz) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < p * k;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    y32[e] = (float)s_yp[e];
-    if (geographic && e % k == 0) {
-      const int64_t t = e / k;
-      const double la = s_a[t] * kPi / 180.0, lo = s_b[t] * kPi / 180.0;
-      ux[t] = cos(la) * cos(lo); uy[t] = cos(la) * sin(lo); uz[t] = sin(la);
-    }
-  }
-}
-
-void launch_mixed_obs(hipStream_t s, int64_t p, int k, const double* s_yp, const double* s_a,
-                      const double* s_b, int geographic, float* y32, double* ux, double* uy,
-                      double* uz) {
-  if (p <= 0) return;
-  hipLaunchKernelGGL(k_mixed_obs, dim3(grid_for(p * k, 256, 65536)), dim3(256), 0, s, p, k,
-                     s_yp, s_a, s_b, geographic, y32, ux, uy, uz);
-  MDA_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------------------

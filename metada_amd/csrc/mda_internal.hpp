@@ -141,11 +141,14 @@ void launch_sumsq(hipStream_t s, const double* X, int64_t count, double* partial
                   int nblocks);
 void launch_dev_sumsq(hipStream_t s, const double* X, int k, int64_t n, double scale,
                       double* partial, int nblocks);
-// observation space: Y' obs-major [p][k], ybar, d; optional member-major Y'
+// observation space: Y' obs-major [p][k], ybar, d; optional member-major Y'.
+// With `order`, position t of every output holds observation order[t] (the
+// bucket order), and y32 (if not null) receives an FP32 copy of Y'.
 void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t p,
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
-                      double* d, double* yp_mm);
+                      double* d, double* yp_mm, const uint32_t* order = nullptr,
+                      float* y32 = nullptr);
 // global (all-observation) Gram: G[a][c] = sum_o r_o Y'[o][a] Q[o][c] with Q =
 // [Y' | d] (mode 0) or D = d + sigma Z - Y' (mode 1, EnKF); partials reduced.
 void launch_global_gram(hipStream_t s, const double* yp_om, const double* d,
@@ -184,16 +187,16 @@ void radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, uint32_t* k
                       int64_t hist_cap);
 void launch_bucket_ranges(hipStream_t s, const uint32_t* sorted_keys, int64_t p,
                           int nbuckets, int* start);
-void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, int k,
-                       const double* yp_om, const double* d, const double* var,
+// per-observation data in bucket order (1/var, coordinates; unit vectors of
+// GEOGRAPHIC locations when ux is not null)
+void launch_gather_obs(hipStream_t s, const uint32_t* order, int64_t p, const double* var,
                        const double* oa, const double* ob, const double* oc, int rweight,
-                       double* s_yp, double* s_d, double* s_rinv, double* s_a, double* s_b,
-                       double* s_c);
-// FP32 copy of the sorted Y' and unit vectors of sorted GEOGRAPHIC locations
-// (lat = s_a, lon = s_b in degrees), for the FP32 / BF16x3 Gram
-void launch_mixed_obs(hipStream_t s, int64_t p, int k, const double* s_yp, const double* s_a,
-                      const double* s_b, int geographic, float* y32, double* ux, double* uy,
-                      double* uz);
+                       double* s_rinv, double* s_a, double* s_b, double* s_c, double* ux,
+                       double* uy, double* uz);
+// Y' rows and d in bucket order, plus the FP32 copy of Y' when y32 is not null
+void launch_gather_rows(hipStream_t s, const uint32_t* order, int64_t p, int k,
+                        const double* yp_om, const double* d, double* s_yp, double* s_d,
+                        float* y32);
 struct letkf_gram_args {
   int k;
   int64_t ncol, nx;

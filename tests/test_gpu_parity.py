@@ -344,6 +344,37 @@ def test_letkf_mixed_precision_contractions(mda, ctx, orc, prec, bound, k):
     assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 10 * bound * inc
 
 
+@pytest.mark.parametrize("prec,bound", [("fp64", TOL), ("bf16x3", 1e-4)])
+def test_letkf_supplied_hx_matches_stencil(mda, ctx, orc, prec, bound):
+    """H(x) handed over member-major (read in input order, rows moved into
+    bucket order afterwards) and the CSR stencil (statistics computed in
+    bucket order) give the same local sets and the oracle's analysis."""
+    k = 40
+    case = latlon_case(k=k, p=5000, seed=91, nmodes=96)
+    code = {"fp64": mda._lib.PREC_FP64, "bf16x3": mda._lib.PREC_BF16X3}[prec]
+    hx = case["H"].apply_host(case["X"])
+    out = {}
+    for name, H in (("stencil", case["H"]), ("hx", mda.ObsOperator(hx=hx))):
+        ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+        obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                              coord=mda._lib.COORD_GEOGRAPHIC)
+        geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+        f = mda.LETKF(ens, obs, H, mda.Config({"inflation": 1.05,
+                                               "localization_radius": 1800.0}),
+                      geometry=geom, ctx=ctx, precision=code)
+        f.Analyse()
+        out[name] = (ens.data.copy(), f.local_counts())
+    prob = orc.Problem(k, case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, case["X"], hx)
+    np.testing.assert_array_equal(out["stencil"][1], out["hx"][1])
+    for name in ("stencil", "hx"):
+        Xm = out[name][0]
+        rms = np.sqrt(np.mean((Xm - Xo) ** 2)) / np.sqrt(np.mean(Xo ** 2))
+        assert rms < bound, (name, rms)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
 def test_letkf_column_passes_match_single_pass(mda, ctx, orc, prec, monkeypatch):
     """Gram -> solve -> transform over passes of 333 columns (the last one
