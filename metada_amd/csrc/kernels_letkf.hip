@@ -652,26 +652,39 @@ __device__ inline double readlane_f64(double v, int l) {
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-constexpr int GM_LD = 40;  // bf16 row stride of the staging tile (32 + 8 pad)
-constexpr int GM_LDF = 36; // float row stride (32 + 4 pad)
+// Row buffer of one batch: row j (observation j of the batch, KM members in
+// FP32) starts at word gm_row_base(j).  The operand reads of a step take
+// lanes (i16, kq) to rows 8kq + e (BF16x3) or 4s + kq (FP32) at member
+// 16t + i16; the row shifts put those 64 reads in 64 different banks.
+template <int KM, int PREC>
+__host__ __device__ constexpr int gm_row_base(int j) {
+  return PREC == MDA_PREC_BF16X3 ? j * KM + 16 * (j >> 3) : j * (KM + 16);
+}
+template <int KM, int PREC>
+__host__ __device__ constexpr int gm_row_words() {
+  return gm_row_base<KM, PREC>(31) + KM;
+}
 
-// KT > 4 (64 < k <= 128): lane m handles members m and m + 64 (MB = 2), the
-// 36 lower tiles stay in one wave's registers (one wave per SIMD).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// KT > 4 (64 < k <= 128): rows carry 128 members (two global_load_lds per
+// row), the 36 lower tiles stay in one wave's registers (one wave per SIMD).
 template <int KT, int PREC>
 __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   constexpr int NT = KT * (KT + 1) / 2;
-  constexpr int MB = KT > 4 ? 2 : 1;  // members per lane
+  constexpr int MB = KT > 4 ? 2 : 1;  // 64-member blocks per row
   constexpr int KM = 64 * MB;         // member capacity
+  constexpr int RW = gm_row_words<KM, PREC>();
   __shared__ int q_off[4][GM_Q];
   __shared__ float q_wd[4][GM_Q];   // w_o d_o (for b)
   __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
   __shared__ int rtab[4][RT_ROWS + 1][5];
-  // staging: BF16x3 uses hi [KM][GM_LD] + lo [KM][GM_LD] bf16; FP32 [KM][GM_LDF]
-  constexpr int STG_BYTES = PREC == MDA_PREC_BF16X3 ? 2 * KM * GM_LD * 2 : KM * GM_LDF * 4;
-  __shared__ __attribute__((aligned(16))) unsigned char stg_raw[4][STG_BYTES];
-  // rows of the batch in flight: [32 observations][KM members] FP32, filled
-  // by global_load_lds (no registers held while the rows are in flight)
-  __shared__ __attribute__((aligned(16))) float rows_raw[4][32 * KM];
+  // scale and weight of the batch in flight, entry j at [j]
+  __shared__ __attribute__((aligned(16))) float bt_sw[4][32];
+  __shared__ __attribute__((aligned(16))) float bt_wd[4][32];
+  // rows of the batch in flight, filled by global_load_lds (no registers
+  // held while the rows are in flight)
+  __shared__ __attribute__((aligned(16))) float rows_raw[4][RW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int k = a.k;
@@ -683,7 +696,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   // XCD-contiguous virtual block index (gridDim.x is a multiple of 8)
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
-  unsigned char* stg = stg_raw[wv];
+  float* rows = rows_raw[wv];
   int64_t it = a.queue ? queue_next(a.queue) : vb * 4 + wv;
   for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + nb * 4) {
     const int64_t g = a.order ? (int64_t)a.order[it] : it;
@@ -692,9 +705,11 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
     f4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-    double bacc[MB];
+    // b partials: lane (i16, kq) sums member 16t + i16 over its observation
+    // slots; the four kq slots are added at the end
+    double bacc[KT];
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) bacc[mb] = 0.0;
+    for (int t = 0; t < KT; ++t) bacc[t] = 0.0;
     int nl = 0, pend = 0;
     double vx = 0.0, vy = 0.0, vz = 0.0;
     if (a.coord == MDA_COORD_GEOGRAPHIC) {
@@ -703,27 +718,30 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
     }
     // Batches of 32 queued observations.  issue(cnt): the rows of queue
     // entries [0, cnt) start towards the wave's LDS row buffer (one
-    // global_load_lds per row: lane m moves member m), and their scale and
-    // weight stay in registers (lane j holds entry j).  consume(): waits for
+    // global_load_lds per row and 64-member block: lane m moves member m),
+    // their scale and weight go to the batch arrays.  consume(): waits for
     // the rows, then scales, splits and multiplies them.  The rows of a batch
     // are in flight while the search goes on; entries j >= cnt carry zero
     // scale and weight.
-    float* rows = rows_raw[wv];
-    float b_wd = 0.f, b_sw = 0.f;
     bool inflight = false;
     auto issue = [&](int cnt) {
       const int my_o = lane < cnt ? q_off[wv][lane] : 0;
-      b_wd = lane < cnt ? q_wd[wv][lane] : 0.f;
-      b_sw = lane < cnt ? q_sw[wv][lane] : 0.f;
+      if (lane < 32) {
+        bt_wd[wv][lane] = lane < cnt ? q_wd[wv][lane] : 0.f;
+        bt_sw[wv][lane] = lane < cnt ? q_sw[wv][lane] : 0.f;
+      }
+      // row base uniform (scalar address), member offset per lane
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
-        const float* ycol = a.y32 + (64 * mb + lane < k ? 64 * mb + lane : 0);
+        const unsigned lm = 64 * mb + lane < k ? 64 * mb + lane : 0;
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
           const int o = __builtin_amdgcn_readlane(my_o, j);
+          const float* yrow = a.y32 + (size_t)o * k;
           __builtin_amdgcn_global_load_lds(
-              (const void*)(ycol + (size_t)o * k),
-              (__attribute__((address_space(3))) void*)(rows + KM * j + 64 * mb), 4, 0, 0);
+              (const void*)(yrow + lm),
+              (__attribute__((address_space(3))) void*)(rows + gm_row_base<KM, PREC>(j) + 64 * mb),
+              4, 0, 0);
         }
       }
       inflight = true;
@@ -734,57 +752,46 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
       __builtin_amdgcn_s_waitcnt(0);
       wave_sync_lds();
       inflight = false;
-      const float my_wd = b_wd, my_sw = b_sw;
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-      const int m = 64 * mb + lane;  // this lane's member in block mb
-      float bp0 = 0.f, bp1 = 0.f;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float av[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int j = 8 * c + e;
-          const float sw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_sw), j));
-          const float wd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_wd), j));
-          const float y = m < k ? rows[KM * j + m] : 0.f;
-          if (e & 1) bp1 = fmaf(wd, y, bp1);
-          else bp0 = fmaf(wd, y, bp0);
-          av[e] = sw * y;
+      if (PREC == MDA_PREC_BF16X3) {
+        // A operand of v_mfma_f32_16x16x32_bf16: lane (i16, kq) holds
+        // A[i16][8kq + e], here member 16t + i16 of observation 8kq + e
+        float sw8[8], wd8[8];
+        {
+          const float4 s0 = *(const float4*)&bt_sw[wv][8 * kq];
+          const float4 s1 = *(const float4*)&bt_sw[wv][8 * kq + 4];
+          const float4 w0 = *(const float4*)&bt_wd[wv][8 * kq];
+          const float4 w1 = *(const float4*)&bt_wd[wv][8 * kq + 4];
+          sw8[0] = s0.x; sw8[1] = s0.y; sw8[2] = s0.z; sw8[3] = s0.w;
+          sw8[4] = s1.x; sw8[5] = s1.y; sw8[6] = s1.z; sw8[7] = s1.w;
+          wd8[0] = w0.x; wd8[1] = w0.y; wd8[2] = w0.z; wd8[3] = w0.w;
+          wd8[4] = w1.x; wd8[5] = w1.y; wd8[6] = w1.z; wd8[7] = w1.w;
         }
-        if (PREC == MDA_PREC_BF16X3) {
+        bf8 fh[KT], fl[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          const int m = 16 * t + i16;
+          float av[8];
+          float bp = 0.f;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float y = m < k ? rows[gm_row_base<KM, PREC>(8 * kq + e) + m] : 0.f;
+            bp = fmaf(wd8[e], y, bp);
+            av[e] = sw8[e] * y;
+          }
+          bacc[t] += (double)bp;
           // a = hi + lo with hardware round-to-nearest-even conversions
-          uint32_t ph[4], pl[4];
+          u32x4 ph, pl;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const fl2 f = {av[2 * q], av[2 * q + 1]};
             const bf2 h = __builtin_convertvector(f, bf2);
             const fl2 r = f - __builtin_convertvector(h, fl2);
             const bf2 lw = __builtin_convertvector(r, bf2);
-            ph[q] = *(const uint32_t*)&h;
-            pl[q] = *(const uint32_t*)&lw;
+            ph[q] = __builtin_bit_cast(uint32_t, h);
+            pl[q] = __builtin_bit_cast(uint32_t, lw);
           }
-          uint16_t* hi = (uint16_t*)stg;
-          uint16_t* lo = hi + KM * GM_LD;
-          *(uint4*)(hi + m * GM_LD + 8 * c) = uint4{ph[0], ph[1], ph[2], ph[3]};
-          *(uint4*)(lo + m * GM_LD + 8 * c) = uint4{pl[0], pl[1], pl[2], pl[3]};
-        } else {
-          float* f = (float*)stg;
-          *(float4*)(f + m * GM_LDF + 8 * c) = float4{av[0], av[1], av[2], av[3]};
-          *(float4*)(f + m * GM_LDF + 8 * c + 4) = float4{av[4], av[5], av[6], av[7]};
-        }
-      }
-      bacc[mb] += (double)bp0 + (double)bp1;
-      }
-      if (PREC == MDA_PREC_BF16X3) {
-        const uint16_t* hi = (const uint16_t*)stg;
-        const uint16_t* lo = hi + KM * GM_LD;
-        wave_sync_lds();
-        bf8 fh[KT], fl[KT];
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-          fh[t] = *(const bf8*)(hi + (16 * t + i16) * GM_LD + 8 * kq);
-          fl[t] = *(const bf8*)(lo + (16 * t + i16) * GM_LD + 8 * kq);
+          fh[t] = __builtin_bit_cast(bf8, ph);
+          fl[t] = __builtin_bit_cast(bf8, pl);
         }
         int tile = 0;
 #pragma unroll
@@ -798,13 +805,23 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
           }
         }
       } else {
-        const float* f = (const float*)stg;
-        wave_sync_lds();
+        // A operand of v_mfma_f32_16x16x4f32: lane (i16, kq) holds A[i16][kq],
+        // here member 16t + i16 of observation 4s + kq
+        float bp[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) bp[t] = 0.f;
 #pragma unroll
         for (int s4 = 0; s4 < 8; ++s4) {
+          const int j = 4 * s4 + kq;
+          const float sw = bt_sw[wv][j], wd = bt_wd[wv][j];
           float fr[KT];
 #pragma unroll
-          for (int t = 0; t < KT; ++t) fr[t] = f[(16 * t + i16) * GM_LDF + 4 * s4 + kq];
+          for (int t = 0; t < KT; ++t) {
+            const int m = 16 * t + i16;
+            const float y = m < k ? rows[gm_row_base<KM, PREC>(j) + m] : 0.f;
+            bp[t] = fmaf(wd, y, bp[t]);
+            fr[t] = sw * y;
+          }
           int tile = 0;
 #pragma unroll
           for (int ta = 0; ta < KT; ++ta) {
@@ -815,6 +832,8 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
             }
           }
         }
+#pragma unroll
+        for (int t = 0; t < KT; ++t) bacc[t] += (double)bp[t];
       }
       wave_sync_lds();
     };
@@ -923,7 +942,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
     //   row i of tile (ta, tb <= ta): the 16 lanes of one kq group, summed by
     //   xor-shuffles over i16; the mirrored upper part of row i is column i
     //   of the tiles below the diagonal, summed over kq and r.
-    double* rsum = (double*)stg;  // [KM] row sums (staging is free now)
+    double* rsum = (double*)rows;  // [KM] row sums (the row buffer is free now)
     {
       float rowp[KT][4], colp[KT];
 #pragma unroll
@@ -966,23 +985,33 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
       }
       wave_sync_lds();
     }
-    double tsum = 0.0, bsum = 0.0;
+    double tsum = 0.0;
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      if (64 * mb + lane < k) {
-        tsum += rsum[64 * mb + lane];
-        bsum += bacc[mb];
-      }
-    }
+    for (int mb = 0; mb < MB; ++mb)
+      if (64 * mb + lane < k) tsum += rsum[64 * mb + lane];
     for (int off = 32; off > 0; off >>= 1) tsum += __shfl_xor(tsum, off);
     const double rk = 1.0 / (double)k;
     const double tot = tsum * rk * rk;
-    for (int off = 32; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
-    // b: lane m holds b_m (and b_(m+64))
+    // b_m: the four observation slots of member m (lanes i16 + 16 kq) added;
+    // members >= k saw zero rows
+    double bsum = 0.0;
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-      if (64 * mb + lane < k) a.b[(size_t)g * k + 64 * mb + lane] = bacc[mb] - bsum * rk;
+    for (int t = 0; t < KT; ++t) {
+      bacc[t] += __shfl_xor(bacc[t], 16);
+      bacc[t] += __shfl_xor(bacc[t], 32);
+      bsum += bacc[t];
+    }
+    for (int off = 8; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
+    if (kq == 0) {
+#pragma unroll
+      for (int t = 0; t < KT; ++t)
+        if (16 * t + i16 < k) a.b[(size_t)g * k + 16 * t + i16] = bacc[t] - bsum * rk;
+    }
     double* Cg = a.C + (size_t)g * k * k;
+    // k through an opaque copy: the element offsets below are recomputed per
+    // column instead of being kept live across the candidate loop
+    int ko = k;
+    __asm__ volatile("" : "+s"(ko));
     int tile = 0;
 #pragma unroll
     for (int ta = 0; ta < KT; ++ta) {
@@ -991,16 +1020,16 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 16 * ta + 4 * kq + r, col = 16 * tb + i16;
-          if (row < k && col < k) {
+          if (row < ko && col < ko) {
             const double v = (double)acc[tile][r] - (rsum[row] + rsum[col]) * rk + tot;
-            Cg[(size_t)row * k + col] = v;
-            if (ta != tb) Cg[(size_t)col * k + row] = v;
+            Cg[row * ko + col] = v;
+            if (ta != tb) Cg[col * ko + row] = v;
           }
         }
         ++tile;
       }
     }
-    wave_sync_lds();  // rsum reads done before the next column's staging
+    wave_sync_lds();  // rsum reads done before the next column's batches
     if (lane == 0) a.nloc[g] = nl;
   }
 }
