@@ -560,6 +560,7 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   }
   const int64_t nck = P.chunk;
   ga.C = c->get<double>(pre + "col_C", (size_t)nck * k * k);
+  if (prm->precision != MDA_PREC_FP64) ga.crec = c->get<int>(pre + "col_rec", (size_t)nck * 96);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
   P.queue = queue;

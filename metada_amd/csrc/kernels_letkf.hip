@@ -430,9 +430,9 @@ __device__ inline void fill_row_table(const letkf_gram_args& a, const col_window
   wave_sync_lds();
 }
 
-__device__ inline row_ranges row_from_table(const letkf_gram_args& a, const col_window& cw,
-                                            int row, const int (*tab)[5]) {
-  const int i = row - cw.row0;
+__device__ inline row_ranges row_from_table(const letkf_gram_args& a, int row0, int row,
+                                            const int (*tab)[5]) {
+  const int i = row - row0;
   row_ranges rr;
   if (i < RT_ROWS) {
     rr.n = tab[i][0];
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
     const double ca = cw.ca, cb = cw.cb, cc = cw.cc;
     const int row0 = cw.row0, row1 = cw.row1;
     for (int row = row0; row <= row1; ++row) {
-      const row_ranges rr = row_from_table(a, cw, row, rtab[wv]);
+      const row_ranges rr = row_from_table(a, cw.row0, row, rtab[wv]);
       for (int sg = 0; sg < rr.n; ++sg) {
         const int64_t s0 = sg ? rr.s0[1] : rr.s0[0];
         const int64_t s1 = sg ? rr.s1[1] : rr.s1[0];
@@ -652,6 +652,35 @@ __device__ inline double readlane_f64(double v, int l) {
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+// Per-column setup of the FP32 / BF16x3 Gram, one wave per column, done in a
+// kernel of its own so that the window's trigonometry (and its constants)
+// stays out of the Gram's registers.  Record of column g, CT_INTS ints:
+//   [0, 6)   reference point, 3 doubles: the unit vector of the column's
+//            location (GEOGRAPHIC) or its coordinates (GRID / CARTESIAN)
+//   [6], [7] first and last bucket row of the window
+//   [8, 93)  the row table (RT_ROWS + 1 entries of 5 ints, as fill_row_table)
+constexpr int CT_INTS = 96;
+
+__global__ __launch_bounds__(256) void k_column_setup(letkf_gram_args a, int* __restrict__ rec) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
+    const col_window cw = column_window(a, g);
+    int* r = rec + (size_t)g * CT_INTS;
+    fill_row_table(a, cw, (int (*)[5])(r + 8));
+    if (lane == 0) {
+      double* ref = (double*)r;
+      if (a.coord == MDA_COORD_GEOGRAPHIC) {
+        const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
+        ref[0] = cos(la) * cos(lo); ref[1] = cos(la) * sin(lo); ref[2] = sin(la);
+      } else {
+        ref[0] = cw.ca; ref[1] = cw.cb; ref[2] = cw.cc;
+      }
+      r[6] = cw.row0;
+      r[7] = cw.row1;
+    }
+  }
+}
+
 // Row buffer of one batch: row j (observation j of the batch, KM members in
 // FP32) starts at word gm_row_base(j).  The operand reads of a step take
 // lanes (i16, kq) to rows 8kq + e (BF16x3) or 4s + kq (FP32) at member
@@ -678,7 +707,8 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   __shared__ int q_off[4][GM_Q];
   __shared__ float q_wd[4][GM_Q];   // w_o d_o (for b)
   __shared__ float q_sw[4][GM_Q];   // sqrt(w_o) (row scale)
-  __shared__ int rtab[4][RT_ROWS + 1][5];
+  // the column's setup record (k_column_setup)
+  __shared__ __attribute__((aligned(16))) int crec[4][CT_INTS];
   // scale and weight of the batch in flight, entry j at [j]
   __shared__ __attribute__((aligned(16))) float bt_sw[4][32];
   __shared__ __attribute__((aligned(16))) float bt_wd[4][32];
@@ -700,8 +730,16 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   int64_t it = a.queue ? queue_next(a.queue) : vb * 4 + wv;
   for (; it < a.ncol; it = a.queue ? queue_next(a.queue) : it + nb * 4) {
     const int64_t g = a.order ? (int64_t)a.order[it] : it;
-    const col_window cw = column_window(a, g);
-    fill_row_table(a, cw, rtab[wv]);
+    {
+      const int2* src = (const int2*)(a.crec + (size_t)g * CT_INTS);
+      if (lane < CT_INTS / 2) ((int2*)crec[wv])[lane] = src[lane];
+      wave_sync_lds();
+    }
+    const int (*rtab)[5] = (const int (*)[5])(crec[wv] + 8);
+    const int row0 = crec[wv][6], row1 = crec[wv][7];
+    const double rx = ((const double*)crec[wv])[0];
+    const double ry = ((const double*)crec[wv])[1];
+    const double rz = ((const double*)crec[wv])[2];
     f4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
@@ -711,11 +749,6 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
 #pragma unroll
     for (int t = 0; t < KT; ++t) bacc[t] = 0.0;
     int nl = 0, pend = 0;
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    if (a.coord == MDA_COORD_GEOGRAPHIC) {
-      const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
-      vx = cos(la) * cos(lo); vy = cos(la) * sin(lo); vz = sin(la);
-    }
     // Batches of 32 queued observations.  issue(cnt): the rows of queue
     // entries [0, cnt) start towards the wave's LDS row buffer (one
     // global_load_lds per row and 64-member block: lane m moves member m),
@@ -853,9 +886,9 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
         pend = rest;
       }
     };
-    for (int row = cw.row0; row <= cw.row1; ++row) {
+    for (int row = row0; row <= row1; ++row) {
       // this row's observation ranges (no dynamically indexed arrays)
-      const row_ranges rr = row_from_table(a, cw, row, rtab[wv]);
+      const row_ranges rr = row_from_table(a, row0, row, rtab);
       for (int sg = 0; sg < rr.n; ++sg) {
         const int64_t s0 = sg ? rr.s0[1] : rr.s0[0];
         const int64_t s1 = sg ? rr.s1[1] : rr.s1[0];
@@ -889,15 +922,15 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
             // the contraction it feeds
             float dist;
             if (geo) {
-              const double dx = vx - ca, dy = vy - cb, dz = vz - cc;
+              const double dx = rx - ca, dy = ry - cb, dz = rz - cc;
               const double c2 = dx * dx + dy * dy + dz * dz;
               in = c2 <= chord2_r;
               dist = 2.f * (float)kEarthRadiusKm * asinf(fminf(1.f, 0.5f * sqrtf((float)c2)));
             } else {
-              const double dx = cw.ca - ca, dy = cw.cb - cb;
+              const double dx = rx - ca, dy = ry - cb;
               double d2 = dx * dx + dy * dy;
               if (a.coord == MDA_COORD_CARTESIAN) {
-                const double dz = cw.cc - cc;
+                const double dz = rz - cc;
                 d2 += dz * dz;
               }
               in = sqrt(d2) <= a.radius;
@@ -943,6 +976,11 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
     //   xor-shuffles over i16; the mirrored upper part of row i is column i
     //   of the tiles below the diagonal, summed over kq and r.
     double* rsum = (double*)rows;  // [KM] row sums (the row buffer is free now)
+    // lane coordinates through opaque copies: the element indices of this
+    // epilogue are recomputed per column instead of being kept live across
+    // the candidate loop
+    int kq_e = kq, i16_e = i16;
+    __asm__ volatile("" : "+v"(kq_e), "+v"(i16_e));
     {
       float rowp[KT][4], colp[KT];
 #pragma unroll
@@ -972,16 +1010,16 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
         colp[t] += __shfl_xor(colp[t], 16);
         colp[t] += __shfl_xor(colp[t], 32);
       }
-      if (i16 == 0) {
+      if (i16_e == 0) {
 #pragma unroll
         for (int t = 0; t < KT; ++t)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) rsum[16 * t + 4 * kq + r] = (double)rowp[t][r];
+          for (int r = 0; r < 4; ++r) rsum[16 * t + 4 * kq_e + r] = (double)rowp[t][r];
       }
       wave_sync_lds();
-      if (kq == 0) {
+      if (kq_e == 0) {
 #pragma unroll
-        for (int t = 0; t < KT; ++t) rsum[16 * t + i16] += (double)colp[t];
+        for (int t = 0; t < KT; ++t) rsum[16 * t + i16_e] += (double)colp[t];
       }
       wave_sync_lds();
     }
@@ -1002,10 +1040,10 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
       bsum += bacc[t];
     }
     for (int off = 8; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
-    if (kq == 0) {
+    if (kq_e == 0) {
 #pragma unroll
       for (int t = 0; t < KT; ++t)
-        if (16 * t + i16 < k) a.b[(size_t)g * k + 16 * t + i16] = bacc[t] - bsum * rk;
+        if (16 * t + i16_e < k) a.b[(size_t)g * k + 16 * t + i16_e] = bacc[t] - bsum * rk;
     }
     double* Cg = a.C + (size_t)g * k * k;
     // k through an opaque copy: the element offsets below are recomputed per
@@ -1019,7 +1057,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
       for (int tb = 0; tb <= ta; ++tb) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = 16 * ta + 4 * kq + r, col = 16 * tb + i16;
+          const int row = 16 * ta + 4 * kq_e + r, col = 16 * tb + i16_e;
           if (row < ko && col < ko) {
             const double v = (double)acc[tile][r] - (rsum[row] + rsum[col]) * rk + tot;
             Cg[row * ko + col] = v;
@@ -1035,7 +1073,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
 }
 
 template <int KT, int PREC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_letkf_gram_mixed(letkf_gram_args a) {
   gram_mixed_body<KT, PREC>(a);
 }
@@ -1180,7 +1218,11 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
               "too many columns for the queue");
   int kt = (a.k + 15) / 16;
   if (a.precision != MDA_PREC_FP64) {
-    MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "mixed Gram inputs missing");
+    MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux) && a.crec,
+                "mixed Gram inputs missing");
+    hipLaunchKernelGGL(k_column_setup, dim3(grid_for(a.ncol, 4, 262144)), dim3(256), 0, s, a,
+                       a.crec);
+    MDA_LAUNCH_CHECK();
     nb = (nb + 7) / 8 * 8;  // XCD-contiguous mapping needs a multiple of 8
     const bool bf = a.precision == MDA_PREC_BF16X3;
 #define MDA_GM(KT)                                                                         \

@@ -222,6 +222,9 @@ struct letkf_gram_args {
   double* C;    // out [ncol][k][k]
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
+  // per-column setup records [ncol][96] of the FP32 / BF16x3 Gram (window
+  // rows, row table, reference point), written by the Gram launch itself
+  int* crec;
   // work-queue mode: persistent waves take columns order[0], order[1], ...
   // through the counter `queue` (zeroed before the launch)
   const uint32_t* order;
