@@ -560,7 +560,8 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   }
   const int64_t nck = P.chunk;
   ga.C = c->get<double>(pre + "col_C", (size_t)nck * k * k);
-  if (prm->precision != MDA_PREC_FP64) ga.crec = c->get<int>(pre + "col_rec", (size_t)nck * 96);
+  if (prm->precision != MDA_PREC_FP64)
+    ga.crec = c->get<int>(pre + "col_rec", (size_t)ncol * LETKF_CREC_INTS);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
   P.queue = queue;
@@ -622,6 +623,8 @@ static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
                     const_cast<double*>(ga.ob), const_cast<double*>(ga.oc),
                     const_cast<double*>(ga.ux), const_cast<double*>(ga.uy),
                     const_cast<double*>(ga.uz));
+  // window geometry of every column for the reduced-precision Gram
+  if (P.prm->precision != MDA_PREC_FP64) launch_column_setup(s, ga);
 }
 
 static void enqueue_local(hipStream_t s, letkf_plan& P) {
@@ -651,6 +654,7 @@ static letkf_gram_args gram_pass(const letkf_plan& P, int64_t c0, int64_t nc) {
   if (ga.col_c) ga.col_c += c0;
   ga.b += c0 * P.k;
   ga.nloc += c0;
+  if (ga.crec) ga.crec += c0 * LETKF_CREC_INTS;
   return ga;
 }
 

@@ -659,7 +659,7 @@ __device__ inline double readlane_f64(double v, int l) {
 //            location (GEOGRAPHIC) or its coordinates (GRID / CARTESIAN)
 //   [6], [7] first and last bucket row of the window
 //   [8, 93)  the row table (RT_ROWS + 1 entries of 5 ints, as fill_row_table)
-constexpr int CT_INTS = 96;
+constexpr int CT_INTS = LETKF_CREC_INTS;
 
 __global__ __launch_bounds__(256) void k_column_setup(letkf_gram_args a, int* __restrict__ rec) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1208,6 +1208,14 @@ void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, doub
   MDA_LAUNCH_CHECK();
 }
 
+void launch_column_setup(hipStream_t s, const letkf_gram_args& a) {
+  MDA_REQUIRE(a.crec, "column setup records missing");
+  if (a.ncol <= 0) return;
+  hipLaunchKernelGGL(k_column_setup, dim3(grid_for(a.ncol, 4, 262144)), dim3(256), 0, s, a,
+                     a.crec);
+  MDA_LAUNCH_CHECK();
+}
+
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   MDA_REQUIRE(a.k >= 1 && a.k <= 128, "LETKF Gram supports k <= 128");
   const int npart = a.k > 64 ? 2 : 1;  // waves per column (FP64 Gram)
@@ -1220,9 +1228,6 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   if (a.precision != MDA_PREC_FP64) {
     MDA_REQUIRE(a.y32 && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux) && a.crec,
                 "mixed Gram inputs missing");
-    hipLaunchKernelGGL(k_column_setup, dim3(grid_for(a.ncol, 4, 262144)), dim3(256), 0, s, a,
-                       a.crec);
-    MDA_LAUNCH_CHECK();
     nb = (nb + 7) / 8 * 8;  // XCD-contiguous mapping needs a multiple of 8
     const bool bf = a.precision == MDA_PREC_BF16X3;
 #define MDA_GM(KT)                                                                         \

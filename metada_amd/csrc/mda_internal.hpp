@@ -223,7 +223,7 @@ struct letkf_gram_args {
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
   // per-column setup records [ncol][96] of the FP32 / BF16x3 Gram (window
-  // rows, row table, reference point), written by the Gram launch itself
+  // rows, row table, reference point), from launch_column_setup
   int* crec;
   // work-queue mode: persistent waves take columns order[0], order[1], ...
   // through the counter `queue` (zeroed before the launch)
@@ -235,6 +235,10 @@ struct letkf_gram_args {
   // matters (grid coordinates of a GRID geometry without coordinates).
   int64_t col0;
 };
+// setup records of columns [0, a.ncol) for the FP32 / BF16x3 Gram (a.crec),
+// LETKF_CREC_INTS ints per column
+constexpr int LETKF_CREC_INTS = 96;
+void launch_column_setup(hipStream_t s, const letkf_gram_args& a);
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
 // per-column cost estimate (observations in the candidate buckets) as
 // descending-order sort keys, and column ids
