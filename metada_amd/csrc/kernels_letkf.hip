@@ -744,7 +744,10 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
     // b partials: lane (i16, kq) sums member 16t + i16 over its observation
-    // slots; the four kq slots are added at the end
+    // slots; the four kq slots are added at the end.  Row-buffer entries of
+    // members >= k hold member 0's values (the row loads of lanes past k read
+    // member 0): their products only reach entries of C and b that the
+    // epilogue leaves out.
     double bacc[KT];
 #pragma unroll
     for (int t = 0; t < KT; ++t) bacc[t] = 0.0;
@@ -802,12 +805,14 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
         bf8 fh[KT], fl[KT];
 #pragma unroll
         for (int t = 0; t < KT; ++t) {
+          // one tile's reads at a time (bounded live range of the row values)
+          __asm__ volatile("" ::: "memory");
           const int m = 16 * t + i16;
           float av[8];
           float bp = 0.f;
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
-            const float y = m < k ? rows[gm_row_base<KM, PREC>(8 * kq + e) + m] : 0.f;
+            const float y = rows[gm_row_base<KM, PREC>(8 * kq + e) + m];
             bp = fmaf(wd8[e], y, bp);
             av[e] = sw8[e] * y;
           }
@@ -851,7 +856,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
 #pragma unroll
           for (int t = 0; t < KT; ++t) {
             const int m = 16 * t + i16;
-            const float y = m < k ? rows[gm_row_base<KM, PREC>(j) + m] : 0.f;
+            const float y = rows[gm_row_base<KM, PREC>(j) + m];
             bp[t] = fmaf(wd, y, bp[t]);
             fr[t] = sw * y;
           }
@@ -996,8 +1001,10 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
         for (int tb = 0; tb <= ta; ++tb) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            rowp[ta][r] += acc[tile][r];
-            if (tb != ta) colp[tb] += acc[tile][r];
+            // entries of members >= k hold products of member 0's copy: left out
+            const bool cv = 16 * tb + i16_e < k, rv = 16 * ta + 4 * kq_e + r < k;
+            rowp[ta][r] += cv ? acc[tile][r] : 0.f;
+            if (tb != ta) colp[tb] += rv ? acc[tile][r] : 0.f;
           }
           ++tile;
         }
@@ -1031,13 +1038,13 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
     const double rk = 1.0 / (double)k;
     const double tot = tsum * rk * rk;
     // b_m: the four observation slots of member m (lanes i16 + 16 kq) added;
-    // members >= k saw zero rows
+    // members >= k left out
     double bsum = 0.0;
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
       bacc[t] += __shfl_xor(bacc[t], 16);
       bacc[t] += __shfl_xor(bacc[t], 32);
-      bsum += bacc[t];
+      if (16 * t + i16_e < k) bsum += bacc[t];
     }
     for (int off = 8; off > 0; off >>= 1) bsum += __shfl_xor(bsum, off);
     if (kq_e == 0) {
