@@ -132,11 +132,38 @@ def main():
         t_sample = time.time() - t1
         per_col = max(t_sample - t_fixed, 1e-9) / len(cols)
         cpu_s = t_fixed + per_col * ncol
+        # one thread on the first 32 sampled columns (the reference is single-threaded)
+        n1 = min(32, len(cols))
+        sel = (np.arange(nlev)[:, None] * len(cols) + np.arange(n1)[None, :]).ravel()
+        prob1 = orc.Problem(k, n1, 1, nlev, orc.COORD_GEOGRAPHIC, col_lat[cols[:n1]],
+                            col_lon[cols[:n1]], W["lat"], W["lon"], yo, var, rho32,
+                            float(np.float32(a.radius)))
+        t1 = time.time()
+        orc.letkf_standard(orc.Problem(k, 1, 1, nlev, orc.COORD_GEOGRAPHIC, col_lat[cols[:1]],
+                                       col_lon[cols[:1]], W["lat"], W["lon"], yo, var, rho32,
+                                       float(np.float32(a.radius))),
+                           Xs[:, ::len(cols)].copy(), Yb, threads=1)
+        t_fixed1 = time.time() - t1
+        t1 = time.time()
+        orc.letkf_standard(prob1, Xs[:, sel].copy(), Yb, threads=1)
+        t_one = time.time() - t1
+        cpu1_s = t_fixed1 + max(t_one - t_fixed1, 1e-9) / n1 * ncol
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh
+                                  if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": 1.0 / cpu_s, "unit": "analyses/s", "cores": threads, "kind": "port",
                "sample": f"{len(cols)} of {ncol} columns (all {nlev} levels, full local-obs "
                          f"search over p={p}), oracle C FP64 standard LETKF with OpenMP; "
                          f"{t_sample:.2f}s for the sample + {t_fixed:.2f}s fixed, "
-                         f"extrapolated to {cpu_s:.0f}s per analysis"}
+                         f"extrapolated to {cpu_s:.0f}s per analysis",
+               "value_1_thread": 1.0 / cpu1_s,
+               "sample_1_thread": f"{n1} columns on 1 thread in {t_one:.2f}s ({t_fixed1:.2f}s "
+                                  f"fixed), extrapolated to {cpu1_s:.0f}s per analysis",
+               "cpu_model": cpu_model, "nproc": os.cpu_count()}
         log(f"oracle sample: {t_sample:.2f}s ({threads} threads), mean local obs "
             f"{nloc_o.mean():.0f}")
     except Exception as e:  # the oracle is a checker; the bench still runs without it
