@@ -98,6 +98,8 @@ __device__ inline double wave_total(double v) {
 // reflector j is stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j,
 // and tau_j in U[j][0] (never a reflector position).
 // ---------------------------------------------------------------------------
+constexpr int E1_CH = 8;  // step chunk of k_trd (static column ranges start at the chunk)
+
 template <int N>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_trd(int64_t ncol, const double* __restrict__ C,
@@ -125,14 +127,14 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
     // registers during the previous step's update (static index, selected
     // on the uniform j), so no row travels through LDS.
     double xc = ar[0];
-    // Steps j in chunks of 16: within chunk ch every quantity indexed by a
+    // Steps j in chunks of E1_CH: within chunk ch every quantity indexed by a
     // column c <= j is zero or unused, so the static column ranges start at
-    // 16 ch (fewer row writes, dot-product and update terms).
-    constexpr int NCH = (N - 2 + 15) / 16;
+    // E1_CH ch (fewer row writes, dot-product and update terms).
+    constexpr int NCH = (N - 2 + E1_CH - 1) / E1_CH;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-      const int c0 = 16 * ch;
-      const int jend = (16 * ch + 16 < n - 2) ? 16 * ch + 16 : n - 2;
+      const int c0 = E1_CH * ch;
+      const int jend = (E1_CH * ch + E1_CH < n - 2) ? E1_CH * ch + E1_CH : n - 2;
 #pragma unroll 1
       for (int j = c0; j < jend; ++j) {
         const double xi = (lane > j && lane < n) ? xc : 0.0;
@@ -164,7 +166,7 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
           ar[c] -= ui * wb[c] + wi * ub[c];
-          if (c > c0 && c <= c0 + 16 && c == j + 1) xc = ar[c];
+          if (c > c0 && c <= c0 + E1_CH && c == j + 1) xc = ar[c];
         }
         wave_lds_sync();
       }
