@@ -98,10 +98,10 @@ __device__ inline double wave_total(double v) {
 // reflector j is stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j,
 // and tau_j in U[j][0] (never a reflector position).
 // ---------------------------------------------------------------------------
-constexpr int E1_CH = 8;  // step chunk of k_trd (static column ranges start at the chunk)
+constexpr int E1_CH = 2;  // step chunk of k_trd (static column ranges start at the chunk)
 
 template <int N>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 void k_trd(int64_t ncol, const double* __restrict__ C,
                                             const int* __restrict__ nloc, double rho,
                                             const double* __restrict__ rho_col, int prior,
