@@ -382,6 +382,8 @@ __device__ inline double fast_div(double a, double b) {
   return a * y;
 }
 
+constexpr int E3_SUB = 4;  // reflectors per static range in the back-transform
+
 template <int N>
 __global__ __launch_bounds__(64) void k_trd_vectors(
     int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
@@ -561,8 +563,8 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
     // (free between the near-pair step and U_s) by global_load_lds, 256
     // bytes per instruction and no registers held, two chunks of 16 rows
     // ahead of their use, and read back as LDS broadcasts.  Reflectors are
-    // taken in chunks of 16 so the static inner range starts at the chunk's
-    // first non-zero position.
+    // applied in sub-chunks of E3_SUB so the static inner range starts at the
+    // sub-chunk's first non-zero position.
     {
       const double* Ucol = Ug + (size_t)g * n * n;
       constexpr int NC = (N - 2 + 15) / 16;
@@ -591,24 +593,31 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
         if (c > 0) wait_vmcnt<N / 2>();
         else wait_vmcnt<0>();
         if (c >= 2) stage(c - 2);
+        // sub-chunks of E3_SUB reflectors, each with its own static range
+#pragma unroll
+        for (int sc = 16 / E3_SUB - 1; sc >= 0; --sc) {
+          const int slo = jlo + E3_SUB * sc;
+          const int shi = (slo + E3_SUB - 1 < jhi) ? slo + E3_SUB - 1 : jhi;
+          if (slo > jhi) continue;
 #pragma unroll 1
-        for (int j = jhi; j >= jlo; --j) {
-          const double* u = Z + j * N;
-          const double tau = u[0];
-          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+          for (int j = shi; j >= slo; --j) {
+            const double* u = Z + j * N;
+            const double tau = u[0];
+            double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-          for (int i = jlo + 1; i < N; ++i) {
-            const double t = u[i] * z[i];
-            switch (i & 3) {
-              case 0: p0 += t; break;
-              case 1: p1 += t; break;
-              case 2: p2 += t; break;
-              default: p3 += t; break;
+            for (int i = slo + 1; i < N; ++i) {
+              const double t = u[i] * z[i];
+              switch (i & 3) {
+                case 0: p0 += t; break;
+                case 1: p1 += t; break;
+                case 2: p2 += t; break;
+                default: p3 += t; break;
+              }
             }
-          }
-          const double dot = tau * ((p0 + p1) + (p2 + p3));
+            const double dot = tau * ((p0 + p1) + (p2 + p3));
 #pragma unroll
-          for (int i = jlo + 1; i < N; ++i) z[i] -= dot * u[i];
+            for (int i = slo + 1; i < N; ++i) z[i] -= dot * u[i];
+          }
         }
       }
     }
