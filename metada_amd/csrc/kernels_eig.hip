@@ -717,7 +717,9 @@ __device__ inline double block2_total(double v, double* red) {
 }
 
 // E1 for N = 128: thread r keeps row r of A in registers.  Per step three
-// block barriers: |x|^2 with the pivot and diagonal, u, and p^T u.
+// block barriers: |x|^2 with the pivot and diagonal, u, and p^T u.  Steps in
+// chunks of E1W_CH (static column ranges start at the chunk).
+constexpr int E1W_CH = 8;
 template <int N>
 __global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
 void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__ nloc,
@@ -738,11 +740,11 @@ void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__
     double* U = Uout + (size_t)g * n * n;
     double* dg = de + (size_t)g * 2 * n;
     double xc = ar[0];  // A[t][j] of the current step
-    constexpr int NCH = (N - 2 + 15) / 16;
+    constexpr int NCH = (N - 2 + E1W_CH - 1) / E1W_CH;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-      const int c0 = 16 * ch;
-      const int jend = (16 * ch + 16 < n - 2) ? 16 * ch + 16 : n - 2;
+      const int c0 = E1W_CH * ch;
+      const int jend = (E1W_CH * ch + E1W_CH < n - 2) ? E1W_CH * ch + E1W_CH : n - 2;
 #pragma unroll 1
       for (int j = c0; j < jend; ++j) {
         const double xi = t > j ? xc : 0.0;
@@ -775,7 +777,7 @@ void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
           ar[c] -= ui * wb[c] + wi * ub[c];
-          if (c > c0 && c <= c0 + 16 && c == j + 1) xc = ar[c];
+          if (c > c0 && c <= c0 + E1W_CH && c == j + 1) xc = ar[c];
         }
       }
     }
