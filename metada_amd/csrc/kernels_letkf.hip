@@ -652,7 +652,7 @@ __device__ inline double readlane_f64(double v, int l) {
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-// Per-column setup of the FP32 / BF16x3 Gram, one wave per column, done in a
+// Per-column setup of the FP32 / BF16x3 Gram, done in a
 // kernel of its own so that the window's trigonometry (and its constants)
 // stays out of the Gram's registers.  Record of column g, CT_INTS ints:
 //   [0, 6)   reference point, 3 doubles: the unit vector of the column's
@@ -661,13 +661,40 @@ __device__ inline double readlane_f64(double v, int l) {
 //   [8, 93)  the row table (RT_ROWS + 1 entries of 5 ints, as fill_row_table)
 constexpr int CT_INTS = LETKF_CREC_INTS;
 
+// One thread per record entry (RT_ROWS row entries, the window entry, the
+// reference point and rows): each thread works out its column's window and
+// then only its own entry.
+constexpr int CT_EPC = RT_ROWS + 2;
+
 __global__ __launch_bounds__(256) void k_column_setup(letkf_gram_args a, int* __restrict__ rec) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < a.ncol; g += (int64_t)gridDim.x * 4) {
+  const bucket_grid& G = a.grid;
+  const int64_t total = a.ncol * CT_EPC;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * 256) {
+    const int64_t g = t / CT_EPC;
+    const int e = (int)(t % CT_EPC);
     const col_window cw = column_window(a, g);
     int* r = rec + (size_t)g * CT_INTS;
-    fill_row_table(a, cw, (int (*)[5])(r + 8));
-    if (lane == 0) {
+    int* tab = r + 8;
+    if (e < RT_ROWS) {
+      // entry e: bucket row row0 + e (entries past the window stay unwritten,
+      // the Gram reads entries of the window's rows only)
+      if (e < cw.row1 - cw.row0 + 1) {
+        const int row = cw.row0 + e;
+        const seg_list rs = row_segments(a, cw, row);
+        tab[5 * e + 0] = rs.n;
+        tab[5 * e + 1] = rs.n > 0 ? a.bstart[row * G.ncolb + rs.j0[0]] : 0;
+        tab[5 * e + 2] = rs.n > 0 ? a.bstart[row * G.ncolb + rs.j1[0] + 1] : 0;
+        tab[5 * e + 3] = rs.n > 1 ? a.bstart[row * G.ncolb + rs.j0[1]] : 0;
+        tab[5 * e + 4] = rs.n > 1 ? a.bstart[row * G.ncolb + rs.j1[1] + 1] : 0;
+      }
+    } else if (e == RT_ROWS) {
+      tab[5 * RT_ROWS + 0] = cw.sl.n;
+      tab[5 * RT_ROWS + 1] = cw.sl.j0[0];
+      tab[5 * RT_ROWS + 2] = cw.sl.j1[0];
+      tab[5 * RT_ROWS + 3] = cw.sl.n > 1 ? cw.sl.j0[1] : 0;
+      tab[5 * RT_ROWS + 4] = cw.sl.n > 1 ? cw.sl.j1[1] : 0;
+    } else {
       double* ref = (double*)r;
       if (a.coord == MDA_COORD_GEOGRAPHIC) {
         const double la = cw.ca * kPi / 180.0, lo = cw.cb * kPi / 180.0;
@@ -1218,8 +1245,8 @@ void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, doub
 void launch_column_setup(hipStream_t s, const letkf_gram_args& a) {
   MDA_REQUIRE(a.crec, "column setup records missing");
   if (a.ncol <= 0) return;
-  hipLaunchKernelGGL(k_column_setup, dim3(grid_for(a.ncol, 4, 262144)), dim3(256), 0, s, a,
-                     a.crec);
+  hipLaunchKernelGGL(k_column_setup, dim3(grid_for(a.ncol * CT_EPC, 256, 262144)), dim3(256), 0,
+                     s, a, a.crec);
   MDA_LAUNCH_CHECK();
 }
 
