@@ -438,6 +438,10 @@ struct letkf_plan {
   int64_t hist_cap = 0;
   double *yp = nullptr, *dd = nullptr;
   uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *hist = nullptr;
+  // processing order of the observation statistics (first stencil tap) and
+  // each observation's position in bucket order
+  uint32_t *tkeys = nullptr, *tvals = nullptr, *ipos = nullptr;
+  int tap_bits = 0;
   int* bstart = nullptr;
   int geo = 0;
   letkf_gram_args ga{};
@@ -528,6 +532,12 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   P.vals2 = c->get<uint32_t>(pre + "bk_vals2", p);
   P.hist_cap = ((p + 255) / 256) * 256;
   P.hist = c->get<uint32_t>(pre + "bk_hist", P.hist_cap);
+  if (!P.h.hx && P.n < ((int64_t)1 << 32)) {
+    P.tkeys = c->get<uint32_t>(pre + "tap_keys", p);
+    P.tvals = c->get<uint32_t>(pre + "tap_vals", p);
+    P.ipos = c->get<uint32_t>(pre + "tap_ipos", p);
+    while (P.tap_bits < 32 && ((int64_t)1 << P.tap_bits) < P.n) ++P.tap_bits;
+  }
   P.bstart = c->get<int>(pre + "bk_start", P.nbuckets + 1);
 
   letkf_gram_args& ga = P.ga;
@@ -608,10 +618,11 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
 }
 
 // Stages before the per-column work (LETKF.hpp:152-211).  The bucket index
-// comes first: the observation statistics then run in bucket order, writing
-// Y', d (and the FP32 Y') straight into the sorted arrays the Gram reads,
-// with the state gathers of neighbouring observations sharing cache lines.
-// A caller-supplied member-major H(x) is read in input order (coalesced) and
+// comes first: the observation statistics then write Y', d (and the FP32 Y')
+// straight into the bucket-ordered arrays the Gram reads.  With a stencil H
+// they process the observations sorted by the grid index of their first tap,
+// so the state gathers of consecutive observations share cache lines.  A
+// caller-supplied member-major H(x) is read in input order (coalesced) and
 // its rows are moved into bucket order afterwards.  Both idempotent.
 static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
   letkf_gram_args& ga = P.ga;
@@ -635,6 +646,16 @@ static void enqueue_local(hipStream_t s, letkf_plan& P) {
                      nullptr, P.yp, P.dd, nullptr);
     launch_gather_rows(s, P.vals, P.p, P.k, P.yp, P.dd, const_cast<double*>(ga.yp),
                        const_cast<double*>(ga.d), y32);
+  } else if (P.tkeys) {
+    // in the order of the first stencil tap (neighbouring state gathers),
+    // each observation's rows written at its bucket position
+    launch_tap_keys(s, P.h.row_ptr, P.h.col, P.p, P.tkeys, P.tvals);
+    radix_sort_pairs(s, P.tkeys, P.tvals, P.keys2, P.vals2, P.p, P.tap_bits, P.hist,
+                     P.hist_cap);
+    launch_invert_order(s, P.vals, P.p, P.ipos);
+    launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, nullptr, P.o.yo,
+                     nullptr, const_cast<double*>(ga.yp), const_cast<double*>(ga.d), nullptr,
+                     P.tvals, y32, P.ipos);
   } else {
     launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, nullptr, P.o.yo,
                      nullptr, const_cast<double*>(ga.yp), const_cast<double*>(ga.d), nullptr,

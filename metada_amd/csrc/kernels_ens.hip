@@ -94,11 +94,14 @@ void launch_dev_sumsq(hipStream_t s, const double* X, int k, int64_t n, double s
 // Observation space (EnKF.hpp:168-184, LETKF.hpp:197-211): hx_j = H x_j for
 // every member (CSR stencil gather, or a caller-supplied member-major H(x)),
 // ybar = mean over members, Y' = hx - ybar, d = y_o - ybar.  A block owns 64
-// output positions; the [64][k] tile is assembled in LDS so that the
-// obs-major Y' rows go out as contiguous, coalesced stores.  With `order`,
-// output position t holds observation order[t]: in the bucket order of the
-// LETKF, the stencils of a block's observations lie close together, so the
-// state gathers of one member share cache lines.
+// observations; the [64][k] tile is assembled in LDS so that the obs-major Y'
+// rows go out as contiguous, coalesced stores.  With `order`, the block's
+// observations are order[t] for its 64 positions t (for the LETKF, sorted by
+// the grid index of their first stencil tap, so that neighbouring
+// observations' state gathers share cache lines); without `opos` observation
+// order[t] is written at position t, with `opos` at position opos[order[t]].
+// Blocks map XCD-contiguously (block i -> slice i mod 8), so consecutive
+// tiles share an L2.
 // ---------------------------------------------------------------------------
 constexpr int OBS_TILE = 64;
 constexpr int OBS_TAPS = 8;  // stencils held in registers (trilinear: 8)
@@ -108,13 +111,18 @@ __global__ __launch_bounds__(256) void k_obs_stats(
     const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col,
     const double* __restrict__ w, const double* __restrict__ hx_mm,
     const double* __restrict__ yo, const uint32_t* __restrict__ order,
-    double* __restrict__ ybar, double* __restrict__ yp_om, double* __restrict__ d,
-    double* __restrict__ yp_mm, float* __restrict__ y32) {
+    const uint32_t* __restrict__ opos, double* __restrict__ ybar,
+    double* __restrict__ yp_om, double* __restrict__ d, double* __restrict__ yp_mm,
+    float* __restrict__ y32) {
   extern __shared__ double tile[];  // [OBS_TILE][k + 1] (padded rows)
   const int kp = k + 1;
   __shared__ double mean_s[OBS_TILE];
+  __shared__ int64_t pos_s[OBS_TILE];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int64_t o0 = (int64_t)blockIdx.x * OBS_TILE;
+  const int64_t nb = gridDim.x;  // a multiple of 8
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t o0 = vb * OBS_TILE;
+  if (o0 >= p) return;
   const int nob = (int)((p - o0) < OBS_TILE ? (p - o0) : OBS_TILE);
   if (hx_mm) {
     // H x_j: entry (ob, j) with ob fastest so hx_mm reads are coalesced
@@ -170,22 +178,25 @@ __global__ __launch_bounds__(256) void k_obs_stats(
     double m = s / (double)k;
     mean_s[ob] = m;
     const int64_t o = order ? (int64_t)order[o0 + ob] : o0 + ob;
-    if (ybar) ybar[o0 + ob] = m;
-    if (d) d[o0 + ob] = yo[o] - m;
+    const int64_t q = opos ? (int64_t)opos[o] : o0 + ob;  // output position
+    pos_s[ob] = q;
+    if (ybar) ybar[q] = m;
+    if (d) d[q] = yo[o] - m;
   }
   __syncthreads();
   for (int e = tid; e < nob * k; e += nt) {
     int ob = e / k, j = e % k;
     double v = tile[ob * kp + j] - mean_s[ob];
-    if (yp_om) yp_om[(size_t)o0 * k + e] = v;
-    if (y32) y32[(size_t)o0 * k + e] = (float)v;
+    const size_t at = (size_t)pos_s[ob] * k + j;
+    if (yp_om) yp_om[at] = v;
+    if (y32) y32[at] = (float)v;
     tile[ob * kp + j] = v;
   }
   if (yp_mm) {
     __syncthreads();
     for (int e = tid; e < nob * k; e += nt) {
       int ob = e % nob, j = e / nob;
-      yp_mm[(size_t)j * p + o0 + ob] = tile[ob * kp + j];
+      yp_mm[(size_t)j * p + pos_s[ob]] = tile[ob * kp + j];
     }
   }
 }
@@ -193,12 +204,50 @@ __global__ __launch_bounds__(256) void k_obs_stats(
 void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t p,
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
-                      double* d, double* yp_mm, const uint32_t* order, float* y32) {
+                      double* d, double* yp_mm, const uint32_t* order, float* y32,
+                      const uint32_t* opos) {
   if (p <= 0) return;
-  unsigned nb = (unsigned)((p + OBS_TILE - 1) / OBS_TILE);
+  const int64_t tiles = (p + OBS_TILE - 1) / OBS_TILE;
+  MDA_REQUIRE(tiles < ((int64_t)1 << 31) - 8, "too many observations");
+  const unsigned nb = (unsigned)((tiles + 7) / 8 * 8);
   size_t shm = sizeof(double) * OBS_TILE * (k + 1);
   hipLaunchKernelGGL(k_obs_stats, dim3(nb), dim3(256), shm, s, X, k, n, p, row_ptr, col, w,
-                     hx_mm, yo, order, ybar, yp_om, d, yp_mm, y32);
+                     hx_mm, yo, order, opos, ybar, yp_om, d, yp_mm, y32);
+  MDA_LAUNCH_CHECK();
+}
+
+// Processing order of the observation statistics: key = grid index of the
+// first stencil tap (0 for an empty stencil), value = observation index.
+__global__ void k_tap_keys(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col,
+                           int64_t p, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < p;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t0 = row_ptr[o];
+    keys[o] = t0 < row_ptr[o + 1] ? (uint32_t)col[t0] : 0u;
+    vals[o] = (uint32_t)o;
+  }
+}
+
+void launch_tap_keys(hipStream_t s, const int64_t* row_ptr, const int64_t* col, int64_t p,
+                     uint32_t* keys, uint32_t* vals) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_tap_keys, dim3(grid_for(p, 256, 65536)), dim3(256), 0, s, row_ptr, col,
+                     p, keys, vals);
+  MDA_LAUNCH_CHECK();
+}
+
+// ipos[order[t]] = t
+__global__ void k_invert_order(const uint32_t* __restrict__ order, int64_t p,
+                               uint32_t* __restrict__ ipos) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < p;
+       t += (int64_t)gridDim.x * blockDim.x)
+    ipos[order[t]] = (uint32_t)t;
+}
+
+void launch_invert_order(hipStream_t s, const uint32_t* order, int64_t p, uint32_t* ipos) {
+  if (p <= 0) return;
+  hipLaunchKernelGGL(k_invert_order, dim3(grid_for(p, 256, 65536)), dim3(256), 0, s, order, p,
+                     ipos);
   MDA_LAUNCH_CHECK();
 }
 

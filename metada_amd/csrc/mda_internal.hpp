@@ -142,13 +142,19 @@ void launch_sumsq(hipStream_t s, const double* X, int64_t count, double* partial
 void launch_dev_sumsq(hipStream_t s, const double* X, int k, int64_t n, double scale,
                       double* partial, int nblocks);
 // observation space: Y' obs-major [p][k], ybar, d; optional member-major Y'.
-// With `order`, position t of every output holds observation order[t] (the
-// bucket order), and y32 (if not null) receives an FP32 copy of Y'.
+// With `order`, observations are processed in the order order[0], order[1],
+// ...; observation order[t] is written at position t, or at opos[order[t]]
+// when `opos` is given.  y32 (if not null) receives an FP32 copy of Y'.
 void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t p,
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
                       double* d, double* yp_mm, const uint32_t* order = nullptr,
-                      float* y32 = nullptr);
+                      float* y32 = nullptr, const uint32_t* opos = nullptr);
+// keys = grid index of each observation's first stencil tap, vals = 0..p-1
+void launch_tap_keys(hipStream_t s, const int64_t* row_ptr, const int64_t* col, int64_t p,
+                     uint32_t* keys, uint32_t* vals);
+// ipos[order[t]] = t
+void launch_invert_order(hipStream_t s, const uint32_t* order, int64_t p, uint32_t* ipos);
 // global (all-observation) Gram: G[a][c] = sum_o r_o Y'[o][a] Q[o][c] with Q =
 // [Y' | d] (mode 0) or D = d + sigma Z - Y' (mode 1, EnKF); partials reduced.
 void launch_global_gram(hipStream_t s, const double* yp_om, const double* d,
