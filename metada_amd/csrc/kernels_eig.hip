@@ -216,17 +216,19 @@ __device__ inline double hyp2(double f, double g) {
   return r;
 }
 
-// r = sqrt(f^2 + g^2) and 1/r from one reciprocal square root (hardware
-// estimate, two Newton steps y <- y (3 - t y^2) / 2): a shorter dependency
-// chain than a square root followed by a division.  Sums of squares outside
-// the normal range (including 0) take the rescaled route.
+// r = sqrt(f^2 + g^2) and 1/r from one reciprocal square root: the hardware
+// estimate y (about 2^-24 relative) refined by one third-order step
+// y <- y (1 + e/2 + 3e^2/8), e = 1 - t y^2 (the Taylor series of
+// (1 - e)^(-1/2)), as accurate as two Newton steps (about 1 ulp, measured
+// on the part) with a shorter dependency chain, and far shorter than a
+// square root followed by a division.  Sums of squares outside the normal
+// range (including 0) take the rescaled route.
 __device__ inline void hyp_rinv(double f, double g, double& r, double& inv) {
   const double t2 = fma(f, f, g * g);
   if (t2 > 1e-290 && t2 < 1e290) {
-    const double h = 0.5 * t2;
     double y = __builtin_amdgcn_rsq(t2);
-    y = fma(y, fma(-h * y, y, 0.5), y);
-    y = fma(y, fma(-h * y, y, 0.5), y);
+    const double e = fma(-t2 * y, y, 1.0);
+    y = fma(y * e, fma(e, 0.375, 0.5), y);
     r = t2 * y;
     inv = y;
   } else {
@@ -372,14 +374,17 @@ __device__ inline double safe_pivot(double v, double tiny) {
   return fabs(v) < tiny ? (v < 0.0 ? -tiny : tiny) : v;
 }
 
-// a / b for the recurrences of the twisted factorisation: hardware
-// reciprocal and two Newton steps (a shorter dependency chain than the IEEE
-// division sequence; about 1 ulp).  b is a safeguarded pivot: |b| >= tiny > 0.
+// a / b for the recurrences of the twisted factorisation: the hardware
+// reciprocal y (about 2^-24 relative) refined by one third-order step,
+// a / b = a y (1 + e + e^2) with e = 1 - b y (about 1 ulp, measured on the
+// part; three dependent operations after the estimate, against five for two
+// Newton steps and far more for the IEEE division sequence).  b is a
+// safeguarded pivot: |b| >= tiny > 0.
 __device__ inline double fast_div(double a, double b) {
-  double y = __builtin_amdgcn_rcp(b);
-  y = fma(y, fma(-b, y, 1.0), y);
-  y = fma(y, fma(-b, y, 1.0), y);
-  return a * y;
+  const double y = __builtin_amdgcn_rcp(b);
+  const double e = fma(-b, y, 1.0);
+  const double q = a * y;
+  return fma(q, fma(e, e, e), q);
 }
 
 constexpr int E3_SUB = 4;  // reflectors per static range in the back-transform
