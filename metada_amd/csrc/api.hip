@@ -552,13 +552,11 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   ga.oc = c->get<double>(pre + "s_c", p);
   ga.radius = prm->radius; ga.localization = prm->localization; ga.precision = prm->precision;
   P.geo = geom->coord == MDA_COORD_GEOGRAPHIC;
-  if (prm->precision != MDA_PREC_FP64) {
-    ga.y32 = c->get<float>(pre + "s_y32", (size_t)p * k);
-    if (P.geo) {
-      ga.ux = c->get<double>(pre + "s_ux", p);
-      ga.uy = c->get<double>(pre + "s_uy", p);
-      ga.uz = c->get<double>(pre + "s_uz", p);
-    }
+  if (prm->precision != MDA_PREC_FP64) ga.y32 = c->get<float>(pre + "s_y32", (size_t)p * k);
+  if (P.geo) {  // unit vectors of the observation locations (chord distances)
+    ga.ux = c->get<double>(pre + "s_ux", p);
+    ga.uy = c->get<double>(pre + "s_uy", p);
+    ga.uz = c->get<double>(pre + "s_uz", p);
   }
   // Column passes: at k > 64 the per-column k x k buffers of the whole grid
   // would not fit in HBM next to the ensemble (3 x 69 GB at k = 128 and
@@ -570,8 +568,7 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   }
   const int64_t nck = P.chunk;
   ga.C = c->get<double>(pre + "col_C", (size_t)nck * k * k);
-  if (prm->precision != MDA_PREC_FP64)
-    ga.crec = c->get<int>(pre + "col_rec", (size_t)ncol * LETKF_CREC_INTS);
+  ga.crec = c->get<int>(pre + "col_rec", (size_t)ncol * LETKF_CREC_INTS);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
   P.queue = queue;
@@ -634,8 +631,8 @@ static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
                     const_cast<double*>(ga.ob), const_cast<double*>(ga.oc),
                     const_cast<double*>(ga.ux), const_cast<double*>(ga.uy),
                     const_cast<double*>(ga.uz));
-  // window geometry of every column for the reduced-precision Gram
-  if (P.prm->precision != MDA_PREC_FP64) launch_column_setup(s, ga);
+  // window geometry of every column for the Gram
+  launch_column_setup(s, ga);
 }
 
 static void enqueue_local(hipStream_t s, letkf_plan& P) {

@@ -500,12 +500,16 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
   constexpr int NA = (NT + NPART - 1) / NPART;
   __shared__ int q_off[4][64];
   __shared__ double q_w[4][64];
-  __shared__ int rtab[4][RT_ROWS + 1][5];
+  // the column's setup record (k_column_setup)
+  __shared__ __attribute__((aligned(16))) int crec[4][LETKF_CREC_INTS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i16 = lane & 15, kk = lane >> 4;
   const int k = a.k;
-  const bucket_grid& G = a.grid;
   const double cgc = a.radius / 2.0;
+  // chord threshold for GEOGRAPHIC: |u - v|^2 <= 4 sin^2(r / 2R), the
+  // haversine test d <= r on unit vectors
+  const double half_ang = a.radius / (2.0 * kEarthRadiusKm);
+  const double chord2_r = half_ang >= kPi / 2.0 ? 5.0 : 4.0 * sin(half_ang) * sin(half_ang);
   const int64_t nitems = a.ncol * NPART;
   int64_t it = a.queue ? queue_next(a.queue) : (int64_t)blockIdx.x * 4 + wv;
   for (; it < nitems; it = a.queue ? queue_next(a.queue) : it + (int64_t)gridDim.x * 4) {
@@ -518,12 +522,18 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
 #pragma unroll
     for (int t = 0; t < KT; ++t) bp[t] = 0.0;
     int nl = 0;
-    const col_window cw = column_window(a, g);
-    fill_row_table(a, cw, rtab[wv]);
-    const double ca = cw.ca, cb = cw.cb, cc = cw.cc;
-    const int row0 = cw.row0, row1 = cw.row1;
+    {
+      const int2* src = (const int2*)(a.crec + (size_t)g * LETKF_CREC_INTS);
+      if (lane < LETKF_CREC_INTS / 2) ((int2*)crec[wv])[lane] = src[lane];
+      wave_sync_lds();
+    }
+    const int (*rtab)[5] = (const int (*)[5])(crec[wv] + 8);
+    const int row0 = crec[wv][6], row1 = crec[wv][7];
+    const double rx = ((const double*)crec[wv])[0];
+    const double ry = ((const double*)crec[wv])[1];
+    const double rz = ((const double*)crec[wv])[2];
     for (int row = row0; row <= row1; ++row) {
-      const row_ranges rr = row_from_table(a, cw.row0, row, rtab[wv]);
+      const row_ranges rr = row_from_table(a, row0, row, rtab);
       for (int sg = 0; sg < rr.n; ++sg) {
         const int64_t s0 = sg ? rr.s0[1] : rr.s0[0];
         const int64_t s1 = sg ? rr.s1[1] : rr.s1[0];
@@ -534,15 +544,22 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
           if (t < s1) {
             double dist;
             if (a.coord == MDA_COORD_GEOGRAPHIC) {
-              dist = haversine_km(ca, cb, a.oa[t], a.ob[t]);
-            } else if (a.coord == MDA_COORD_CARTESIAN) {
-              double dx = ca - a.oa[t], dy = cb - a.ob[t], dz = cc - a.oc[t];
-              dist = sqrt(dx * dx + dy * dy + dz * dz);
+              // great-circle distance from the chord of unit vectors,
+              // d = 2R asin(|u - v| / 2), equal to the haversine distance
+              const double dx = rx - a.ux[t], dy = ry - a.uy[t], dz = rz - a.uz[t];
+              const double c2 = dx * dx + dy * dy + dz * dz;
+              in = c2 <= chord2_r;
+              dist = 2.0 * kEarthRadiusKm * asin(fmin(1.0, 0.5 * sqrt(c2)));
             } else {
-              double dx = ca - a.oa[t], dy = cb - a.ob[t];
-              dist = sqrt(dx * dx + dy * dy);
+              const double dx = rx - a.oa[t], dy = ry - a.ob[t];
+              double d2 = dx * dx + dy * dy;
+              if (a.coord == MDA_COORD_CARTESIAN) {
+                const double dz = rz - a.oc[t];
+                d2 += dz * dz;
+              }
+              dist = sqrt(d2);
+              in = dist <= a.radius;
             }
-            in = dist <= a.radius;
             if (in)
               w = (a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / cgc) : 1.0) *
                   a.rinv[t];
@@ -592,6 +609,10 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
       bp[ta] += __shfl_xor(bp[ta], 32);
     }
     double* Cg = a.C + (size_t)g * k * k;
+    // k and the lane coordinates through opaque copies: the element offsets
+    // below are recomputed per column, not kept live across the search
+    int ko = k, kk_e = kk, i16_e = i16;
+    __asm__ volatile("" : "+s"(ko), "+v"(kk_e), "+v"(i16_e));
     int tile = 0;
 #pragma unroll
     for (int ta = 0; ta < KT; ++ta) {
@@ -600,10 +621,10 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
         if (tile % NPART == part) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            int row = 16 * ta + kk + 4 * r, col = 16 * tb + i16;
-            if (row < k && col < k) {
-              Cg[(size_t)row * k + col] = acc[tile / NPART][r];
-              if (ta != tb) Cg[(size_t)col * k + row] = acc[tile / NPART][r];
+            int row = 16 * ta + kk_e + 4 * r, col = 16 * tb + i16_e;
+            if (row < ko && col < ko) {
+              Cg[row * ko + col] = acc[tile / NPART][r];
+              if (ta != tb) Cg[col * ko + row] = acc[tile / NPART][r];
             }
           }
         }
@@ -1282,6 +1303,7 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
     MDA_LAUNCH_CHECK();
     return;
   }
+  MDA_REQUIRE(a.crec && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "FP64 Gram inputs missing");
   switch (kt) {
     case 1: hipLaunchKernelGGL((k_letkf_gram_f64<1, 1>), dim3(nb), dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL((k_letkf_gram_f64<2, 1>), dim3(nb), dim3(256), 0, s, a); break;

@@ -222,14 +222,14 @@ struct letkf_gram_args {
   int localization;
   int precision;
   const float* y32;    // sorted obs-major Y' in FP32 (FP32 / BF16x3 contractions)
-  const double* ux;    // sorted unit vectors of GEOGRAPHIC observation locations
+  const double* ux;    // sorted unit vectors of GEOGRAPHIC observation locations (any precision)
   const double* uy;
   const double* uz;
   double* C;    // out [ncol][k][k]
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
-  // per-column setup records [ncol][96] of the FP32 / BF16x3 Gram (window
-  // rows, row table, reference point), from launch_column_setup
+  // per-column setup records [ncol][96] of the Gram (window rows, row table,
+  // reference point), from launch_column_setup
   int* crec;
   // work-queue mode: persistent waves take columns order[0], order[1], ...
   // through the counter `queue` (zeroed before the launch)
@@ -241,7 +241,7 @@ struct letkf_gram_args {
   // matters (grid coordinates of a GRID geometry without coordinates).
   int64_t col0;
 };
-// setup records of columns [0, a.ncol) for the FP32 / BF16x3 Gram (a.crec),
+// setup records of columns [0, a.ncol) for the Gram kernels (a.crec),
 // LETKF_CREC_INTS ints per column
 constexpr int LETKF_CREC_INTS = 96;
 void launch_column_setup(hipStream_t s, const letkf_gram_args& a);
