@@ -495,7 +495,7 @@ void launch_column_cost(hipStream_t s, const letkf_gram_args& a, uint32_t* keys,
 // per column (tile t to wave t % NPART), each repeating the candidate search,
 // so the accumulators of one wave stay within its registers.
 template <int KT, int NPART>
-__global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
+__device__ __forceinline__ void gram_f64_body(const letkf_gram_args& a) {
   constexpr int NT = KT * (KT + 1) / 2;
   constexpr int NA = (NT + NPART - 1) / NPART;
   __shared__ int q_off[4][64];
@@ -641,6 +641,18 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64(letkf_gram_args a) {
     }
     if (lane == 0) a.nloc[g] = nl;
   }
+}
+
+// k <= 64: three waves per SIMD; 64 < k <= 128 (two waves per column, 18
+// tiles each): one.
+template <int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_letkf_gram_f64(letkf_gram_args a) {
+  gram_f64_body<KT, 1>(a);
+}
+
+__global__ __launch_bounds__(256) void k_letkf_gram_f64_w(letkf_gram_args a) {
+  gram_f64_body<8, 2>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1305,12 +1317,12 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   }
   MDA_REQUIRE(a.crec && (a.coord != MDA_COORD_GEOGRAPHIC || a.ux), "FP64 Gram inputs missing");
   switch (kt) {
-    case 1: hipLaunchKernelGGL((k_letkf_gram_f64<1, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_letkf_gram_f64<2, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_letkf_gram_f64<3, 1>), dim3(nb), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_letkf_gram_f64<4, 1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_letkf_gram_f64<1>), dim3(nb), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_letkf_gram_f64<2>), dim3(nb), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_letkf_gram_f64<3>), dim3(nb), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_letkf_gram_f64<4>), dim3(nb), dim3(256), 0, s, a); break;
     default:  // 64 < k <= 128: 8 tile rows, two waves per column
-      hipLaunchKernelGGL((k_letkf_gram_f64<8, 2>), dim3(nb), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_letkf_gram_f64_w, dim3(nb), dim3(256), 0, s, a);
       break;
   }
   MDA_LAUNCH_CHECK();
