@@ -596,6 +596,8 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
                prm->eig_method == MDA_EIG_AUTO;
   sa.U = P.eig_path ? c->get<double>(pre + "eig_U", (size_t)nck * k * k) : nullptr;
   sa.flag = c->get<int>(pre + "eig_flag", ncol);
+  sa.wlist = c->get<int>(pre + "solve_list", ncol);
+  sa.wcount = c->get<unsigned>(pre + "solve_count", 1);
   // Jacobi (fallback, or the Jacobi-only method) beyond the LDS capacity runs
   // in a global per-block scratch
   {

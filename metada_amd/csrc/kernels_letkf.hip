@@ -1338,8 +1338,33 @@ void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
 //   element-wise combination of LETKF.hpp:235-238).
 // A column without local observations gets T = sqrt(rho) I (LETKF.hpp:167-190).
 // ---------------------------------------------------------------------------
+// Columns the fallback solve works on (flagged by the eigensolver, or without
+// local observations), compacted into a list with wave-aggregated atomics.
+// The list's order varies from run to run; every column is solved on its own,
+// so the results do not.
+__global__ void k_solve_count_zero(unsigned* cnt) { *cnt = 0u; }
+
+__global__ void k_solve_worklist(int64_t ncol, const int* __restrict__ only,
+                                 const int* __restrict__ nloc, int* __restrict__ list,
+                                 unsigned* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ncol;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const bool need = only[g] != 0 || nloc[g] == 0;
+    const uint64_t m = __ballot(need);
+    if (m == 0ull) continue;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0u;
+    if (lane == leader) base = atomicAdd(cnt, (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, leader);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (need) list[base + __popcll(m & below)] = (int)g;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double* gscratch,
-                                                     int use_lds, const int* only) {
+                                                     int use_lds, const int* only,
+                                                     const int* list, const unsigned* cnt) {
   extern __shared__ double sh[];
   __shared__ double rc[64], rs[64], red[256], bv[128], yv[128], wb[128];
   __shared__ int rp[64], rq[64], flag;
@@ -1347,8 +1372,11 @@ __global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double*
   const int kp = (k + 1) & ~1, ld = kp + 1;
   double* A = use_lds ? sh : gscratch + (size_t)blockIdx.x * 2 * kp * ld;
   double* V = A + kp * ld;
-  for (int64_t g = blockIdx.x; g < a.ncol; g += gridDim.x) {
-    // fallback mode: flagged columns, plus the empty columns (sqrt(rho) I)
+  // fallback mode: the listed columns (flagged, or empty: sqrt(rho) I);
+  // otherwise every column
+  const int64_t nwork = list ? (int64_t)*cnt : a.ncol;
+  for (int64_t it = blockIdx.x; it < nwork; it += gridDim.x) {
+    const int64_t g = list ? (int64_t)list[it] : it;
     if (only && only[g] == 0 && a.nloc[g] != 0) continue;
     const double rho = a.rho_col ? a.rho_col[g] : a.rho;
     double* Tg = a.T + (size_t)g * k * k;
@@ -1464,13 +1492,26 @@ void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a) {
     if (nb > (unsigned)a.jblocks) nb = (unsigned)a.jblocks;
   }
   const int* only = nullptr;
+  const int* list = nullptr;
+  const unsigned* cnt = nullptr;
   if (a.sqrt_form == MDA_SQRT_SYMMETRIC && letkf_eig_supported(a.k) && a.method == 0) {
     // tridiagonal eigensolver; flagged columns fall back to Jacobi below
     launch_letkf_eig(s, a, a.de, a.lam, a.flag);
     only = a.flag;
+    if (a.wlist && a.wcount) {
+      // the columns to solve, listed, so that an analysis without fallback
+      // columns costs a scan instead of a pass over every column
+      hipLaunchKernelGGL(k_solve_count_zero, dim3(1), dim3(1), 0, s, a.wcount);
+      MDA_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_solve_worklist, dim3(grid_for(a.ncol, 256, 4096)), dim3(256), 0, s,
+                         a.ncol, a.flag, a.nloc, a.wlist, a.wcount);
+      MDA_LAUNCH_CHECK();
+      list = a.wlist;
+      cnt = a.wcount;
+    }
   }
   hipLaunchKernelGGL(k_letkf_solve, dim3(nb), dim3(256), use_lds ? shm : 0, s, a, gscratch,
-                     use_lds, only);
+                     use_lds, only, list, cnt);
   MDA_LAUNCH_CHECK();
 }
 

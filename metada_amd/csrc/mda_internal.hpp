@@ -286,6 +286,10 @@ struct letkf_solve_args {
   // 2 kp (kp + 1) doubles for jblocks blocks
   double* jscratch;
   int jblocks;
+  // fallback work list [ncol] and its length (set: only the flagged and empty
+  // columns are visited after the tridiagonal eigensolver)
+  int* wlist;
+  unsigned* wcount;
   int method;    // 0: tridiagonal + Jacobi fallback, 1: Jacobi only
   hipEvent_t ev_stage[3];  // recorded after E1, E2, E3 when set (timing)
   // Reduced-precision Gram: eigenvalues within null_rel * ||A|| of (k-1)/rho
