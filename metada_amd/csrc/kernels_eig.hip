@@ -165,7 +165,7 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
 #pragma unroll
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
-          ar[c] -= ui * wb[c] + wi * ub[c];
+          ar[c] = fma(-ui, wb[c], fma(-wi, ub[c], ar[c]));  // two FMAs
           if (c > c0 && c <= c0 + E1_CH && c == j + 1) xc = ar[c];
         }
         wave_lds_sync();
@@ -781,7 +781,7 @@ void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__
 #pragma unroll
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
-          ar[c] -= ui * wb[c] + wi * ub[c];
+          ar[c] = fma(-ui, wb[c], fma(-wi, ub[c], ar[c]));  // two FMAs
           if (c > c0 && c <= c0 + E1W_CH && c == j + 1) xc = ar[c];
         }
       }
