@@ -265,10 +265,12 @@ void launch_gather_rows(hipStream_t s, const uint32_t* order, int64_t p, int k,
 
 // ---------------------------------------------------------------------------
 // 2. per-column Gram on FP64 MFMA.  One wavefront per column.  For each
-// bucket range in reach, 64 candidates at a time: distance (Location.hpp
-// metric), localisation weight (GC taper or cutoff) times 1/var, ballot-
-// compaction of the nonzero weights into LDS, then K-steps of 4 observations
-// through v_mfma_f64_16x16x4f64 on the lower 16x16 tiles of C.
+// bucket range in reach (from the column's setup record), 64 candidates at a
+// time: distance (Location.hpp metric; GEOGRAPHIC from the chord of unit
+// vectors, equal to the haversine distance), localisation weight (GC taper
+// or cutoff) times 1/var, ballot-compaction of the nonzero weights into LDS,
+// then K-steps of 4 observations through v_mfma_f64_16x16x4f64 on the lower
+// 16x16 tiles of C.
 //   f64 16x16x4 lane layout: A[l%16][l/16], B[l/16][l%16],
 //   D[l/16 + 4r][l%16] (r = 0..3).
 // ---------------------------------------------------------------------------
@@ -661,13 +663,15 @@ __global__ __launch_bounds__(256) void k_letkf_gram_f64_w(letkf_gram_args a) {
 // GEOGRAPHIC distance from the chord of unit vectors, d = 2R asin(|u-v|/2),
 // equal to the haversine distance).  C = sum_o (sqrt(w_o) y'_o)(sqrt(w_o) y'_o)^T
 // so one scaled copy serves as both MFMA operands.  Compacted observations
-// are queued in LDS and consumed 32 at a time: lane m stages member m of the
-// 32 scaled rows (coalesced FP32 loads) into LDS as [member][obs];
+// are queued in LDS and consumed 32 at a time: their FP32 rows are copied
+// into the wave's LDS row buffer by global_load_lds, and each lane builds its
+// operand fragments straight from that buffer (scaled by sqrt(w_o));
 //   BF16x3: a = hi + lo (bf16, round to nearest even),
 //           C += hi hi^T + hi lo^T + lo hi^T on v_mfma_f32_16x16x32_bf16
 //           (A[l%16][8(l/16)+e], B[8(l/16)+e][l%16], D[4(l/16)+r][l%16]);
 //   FP32:   C += a a^T on v_mfma_f32_16x16x4f32 (A[l%16][l/16]).
-// b = sum_o w_o d_o y'_o accumulates in FP64 (lane m owns b_m).
+// b = sum_o w_o d_o y'_o: FP32 partial sums per lane and batch, accumulated in
+// FP64.  The column's window comes from its setup record (k_column_setup).
 // Blocks are mapped XCD-contiguously: the dispatcher deals blocks to the 8
 // XCDs round-robin, so block i runs columns from slice (i % 8) and columns
 // that share observations share an L2.
