@@ -619,8 +619,8 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
 // Stages before the per-column work (LETKF.hpp:152-211).  The bucket index
 // comes first: the observation statistics then write Y', d (and the FP32 Y')
 // straight into the bucket-ordered arrays the Gram reads.  With a stencil H
-// they process the observations sorted by the grid index of their first tap,
-// so the state gathers of consecutive observations share cache lines.  A
+// they process the observations sorted by the grid position of their first
+// tap, so the state gathers of consecutive observations share cache lines.  A
 // caller-supplied member-major H(x) is read in input order (coalesced) and
 // its rows are moved into bucket order afterwards.  Both idempotent.
 static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
@@ -648,7 +648,7 @@ static void enqueue_local(hipStream_t s, letkf_plan& P) {
   } else if (P.tkeys) {
     // in the order of the first stencil tap (neighbouring state gathers),
     // each observation's rows written at its bucket position
-    launch_tap_keys(s, P.h.row_ptr, P.h.col, P.p, P.tkeys, P.tvals);
+    launch_tap_keys(s, P.h.row_ptr, P.h.col, P.p, P.nx * P.ny, P.tkeys, P.tvals);
     radix_sort_pairs(s, P.tkeys, P.tvals, P.keys2, P.vals2, P.p, P.tap_bits, P.hist,
                      P.hist_cap);
     launch_invert_order(s, P.vals, P.p, P.ipos);

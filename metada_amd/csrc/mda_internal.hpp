@@ -150,9 +150,10 @@ void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t 
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
                       double* d, double* yp_mm, const uint32_t* order = nullptr,
                       float* y32 = nullptr, const uint32_t* opos = nullptr);
-// keys = grid index of each observation's first stencil tap, vals = 0..p-1
+// keys = grid position of each observation's first stencil tap (groups of
+// four levels share a key range; nxy = points per level), vals = 0..p-1
 void launch_tap_keys(hipStream_t s, const int64_t* row_ptr, const int64_t* col, int64_t p,
-                     uint32_t* keys, uint32_t* vals);
+                     int64_t nxy, uint32_t* keys, uint32_t* vals);
 // ipos[order[t]] = t
 void launch_invert_order(hipStream_t s, const uint32_t* order, int64_t p, uint32_t* ipos);
 // global (all-observation) Gram: G[a][c] = sum_o r_o Y'[o][a] Q[o][c] with Q =
