@@ -536,7 +536,9 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
     P.tkeys = c->get<uint32_t>(pre + "tap_keys", p);
     P.tvals = c->get<uint32_t>(pre + "tap_vals", p);
     P.ipos = c->get<uint32_t>(pre + "tap_ipos", p);
-    while (P.tap_bits < 32 && ((int64_t)1 << P.tap_bits) < P.n) ++P.tap_bits;
+    // sort keys below ((nlev - 1) / 4 + 1) nx ny (groups of four levels)
+    const int64_t kmax = ((P.nlev + 3) / 4) * P.nx * P.ny;
+    while (P.tap_bits < 32 && ((int64_t)1 << P.tap_bits) < kmax) ++P.tap_bits;
   }
   P.bstart = c->get<int>(pre + "bk_start", P.nbuckets + 1);
 
