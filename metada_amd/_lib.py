@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmetada_amd.so")
 
 MDA_OK = 0
+MDA_E_ARG, MDA_E_DEVICE, MDA_E_NOMEM, MDA_E_NUMERIC, MDA_E_UNSUPPORTED = -1, -2, -3, -4, -5
 ERRORS = {-1: "MDA_E_ARG", -2: "MDA_E_DEVICE", -3: "MDA_E_NOMEM", -4: "MDA_E_NUMERIC",
           -5: "MDA_E_UNSUPPORTED"}
 

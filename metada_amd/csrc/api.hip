@@ -82,9 +82,11 @@ struct dev_obs {
   const double* c = nullptr;
 };
 
-dev_obs stage_obs(mda_ctx* c, const mda_obs* o, bool need_loc) {
+dev_obs stage_obs(mda_ctx* c, const mda_obs* o, bool need_loc, bool allow_empty = false) {
   MDA_REQUIRE(o != nullptr, "observations are null");
-  MDA_REQUIRE(o->p > 0, "no observations");
+  MDA_REQUIRE(o->p > 0 || (allow_empty && o->p == 0),
+              "no observations (the global EnKF / ETKF need p >= 1)");
+  if (o->p == 0) return dev_obs{};
   MDA_REQUIRE(o->value && o->variance, "observation values and variances are required");
   dev_obs d;
   d.p = o->p;
@@ -145,6 +147,46 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
   return ms;
+}
+
+// Observation values must be finite and variances > 0 (+inf marks an invalid
+// observation, GridObservation.hpp:239-252); a violation sets
+// MDA_ST_NONFINITE_OBS in the status word, so the update is skipped.
+__global__ void k_check_obs(const double* __restrict__ yo, const double* __restrict__ var,
+                            int64_t p, int* __restrict__ status) {
+  bool bad = false;
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < p;
+       o += (int64_t)gridDim.x * blockDim.x)
+    bad = bad || !isfinite(yo[o]) || !(var[o] > 0.0);
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(status, MDA_ST_NONFINITE_OBS);
+}
+
+void launch_check_obs(hipStream_t s, const dev_obs& o, int* status) {
+  if (o.p <= 0) return;
+  hipLaunchKernelGGL(k_check_obs, dim3(grid_for(o.p, 256, 2048)), dim3(256), 0, s, o.yo, o.var,
+                     o.p, status);
+  MDA_LAUNCH_CHECK();
+}
+
+// The analysis's status word as an error (after the stream has drained).
+void throw_status(int st, const char* what) {
+  if (st & MDA_ST_NONFINITE_OBS)
+    throw mda_error(MDA_E_NUMERIC,
+                    std::string(what) + ": non-finite observation value, H(x) or innovation, "
+                    "or an observation variance that is not > 0 (ensemble left unchanged)");
+  if (st & MDA_ST_BOUNDARY_OVERFLOW)
+    throw mda_error(MDA_E_UNSUPPORTED,
+                    std::string(what) + ": more column-observation pairs at the localisation "
+                    "radius (within 1e-9 relative) than the boundary list holds "
+                    "(ensemble left unchanged)");
+  if (st & MDA_ST_NOT_SPD)
+    throw mda_error(MDA_E_NUMERIC, std::string(what) +
+                                       ": a matrix to factor is not positive definite "
+                                       "(ensemble left unchanged)");
+  if (st & MDA_ST_NONFINITE_STATE)
+    throw mda_error(MDA_E_NUMERIC,
+                    std::string(what) + ": non-finite background values; the analysis is "
+                    "non-finite at those state points");
 }
 
 }  // namespace
@@ -257,34 +299,95 @@ int mda_obs_innovations(mda_ctx* c, const mda_ensemble* ens, const mda_obs* obs,
 }
 
 // ---------------------------------------------------------------------------
+// cond(S) of the EnKF, S = Y'Y'^T/(k-1) + diag(var) (EnKF.hpp:205-209,
+// JacobiSVD singular values; S is SPD, so they are its eigenvalues):
+//   p <= 128                  Jacobi on the dense S (one workgroup);
+//   constant var, p > k - 1   S = s^2 (I + Y'Y'^T/(s^2 (k-1))): the smallest
+//                             eigenvalue is s^2 (rank deficiency) and the
+//                             largest s^2 (1 + mu/(k-1)), mu the largest
+//                             eigenvalue of E = Y'^T R^-1 Y' (from [E | b]);
+//   otherwise                 Householder tridiagonalisation of the dense S
+//                             and bisection for its extreme eigenvalues
+//                             (p <= MDA_DENSE_MAX).
+static double enkf_condition_number(mda_ctx* c, const double* yp, const double* var, int k,
+                                    int64_t p, const double* EB) {
+  hipStream_t s = c->stream;
+  double* cnd = c->get<double>("enkf_cond", 1);
+  std::vector<double> hv(p);
+  d2h(c, hv.data(), var, sizeof(double) * p);
+  // an invalid observation (var = +inf) is an infinite singular value of S
+  if (std::any_of(hv.begin(), hv.end(), [](double v) { return std::isinf(v); }))
+    return std::numeric_limits<double>::infinity();
+  if (p <= 128) {
+    double* S = c->get<double>("enkf_S", (size_t)130 * 130);
+    launch_cond_small(s, yp, var, k, (int)p, S, cnd);
+    double v = 0.0;
+    d2h(c, &v, cnd, sizeof(double));
+    return v;
+  }
+  const bool constant = std::all_of(hv.begin(), hv.end(), [&](double v) { return v == hv[0]; });
+  if (constant) {
+    double* S = c->get<double>("enkf_S", (size_t)130 * 130);
+    launch_sym_maxeig(s, EB, k, S, cnd);
+    double mu = 0.0;
+    d2h(c, &mu, cnd, sizeof(double));
+    return 1.0 + mu / (double)(k - 1);
+  }
+  return dense_spd_condition(c, yp, var, k, p);
+}
+
+}  // extern "C"
+
+namespace mda {
+
+int guarded_call(mda_ctx* ctx, const std::function<void()>& f) { return guarded(ctx, f); }
+
+// ETKF (ETKF.hpp:100-178): Y' and d, the global Gram [E | b], the k x k
+// solve and the update, on the device buffers `b` (from the context
+// workspace when null).
+void etkf_run(mda_ctx* c, double inflation, mda_ensemble* ens, const mda_obs* obs,
+              const mda_hop* hop, const etkf_bufs* given) {
+  MDA_REQUIRE(std::isfinite(inflation), "inflation must be finite");
+  dev_ensemble de = stage_ensemble(c, ens);
+  dev_obs o = stage_obs(c, obs, false);
+  dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
+  const int k = de.k;
+  MDA_REQUIRE(k <= 128, "ETKF supports k <= 128");
+  const int64_t p = o.p;
+  etkf_bufs b;
+  if (given) {
+    b = *given;
+  } else {
+    b.yp = c->get<double>("yp_om", (size_t)p * k);
+    b.dd = c->get<double>("d", p);
+    b.part = c->get<double>("gg_part", (size_t)ETKF_GG_BLOCKS * k * (k + 1));
+    b.EB = c->get<double>("gg_EB", (size_t)k * (k + 1));
+    b.T = c->get<double>("gT", (size_t)k * k);
+    b.scr = c->get<double>("g_scr", (size_t)2 * k * k);
+    b.status = c->get<int>("g_status", 1);
+  }
+  hipStream_t s = c->stream;
+  launch_zero(s, b.status, sizeof(int));
+  launch_check_obs(s, o, b.status);
+  launch_obs_stats(s, de.X, k, de.n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, b.yp, b.dd,
+                   nullptr, nullptr, nullptr, nullptr, b.status);
+  launch_global_gram(s, b.yp, b.dd, o.var, nullptr, k, p, 0, b.part, ETKF_GG_BLOCKS, b.EB);
+  launch_etkf_solve_ws(s, k, b.EB, b.T, b.scr, b.status);
+  launch_transform_global(s, de.X, k, de.n, inflation, b.T, b.status);
+  int hs = 0;
+  MDA_HIP(hipMemcpyAsync(&hs, b.status, sizeof(int), hipMemcpyDeviceToHost, s));
+  MDA_HIP(hipStreamSynchronize(s));
+  throw_status(hs, "ETKF");
+  unstage_ensemble(c, ens, de);
+}
+
+}  // namespace mda
+
+extern "C" {
+
 int mda_etkf_analyse(mda_ctx* c, double inflation, mda_ensemble* ens, const mda_obs* obs,
                      const mda_hop* hop) {
-  return guarded(c, [&] {
-    MDA_REQUIRE(std::isfinite(inflation), "inflation must be finite");
-    dev_ensemble de = stage_ensemble(c, ens);
-    dev_obs o = stage_obs(c, obs, false);
-    dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
-    const int k = de.k;
-    const int64_t p = o.p;
-    double* yp = c->get<double>("yp_om", (size_t)p * k);
-    double* dd = c->get<double>("d", p);
-    launch_obs_stats(c->stream, de.X, k, de.n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp,
-                     dd, nullptr);
-    const int nb = 256;
-    double* part = c->get<double>("gg_part", (size_t)nb * k * (k + 1));
-    double* EB = c->get<double>("gg_EB", (size_t)k * (k + 1));
-    launch_global_gram(c->stream, yp, dd, o.var, nullptr, k, p, 0, part, nb, EB);
-    double* T = c->get<double>("gT", (size_t)k * k);
-    double* scr = c->get<double>("g_scr", (size_t)2 * k * k);
-    int* st = c->get<int>("g_status", 1);
-    launch_etkf_solve_ws(c->stream, k, EB, T, scr, st);
-    launch_transform_global(c->stream, de.X, k, de.n, inflation, T);
-    int hs = 0;
-    d2h(c, &hs, st, sizeof(int));
-    MDA_HIP(hipStreamSynchronize(c->stream));
-    if (hs) throw mda_error(MDA_E_NUMERIC, "ETKF: Pa is not positive definite");
-    unstage_ensemble(c, ens, de);
-  });
+  return guarded(c, [&] { etkf_run(c, inflation, ens, obs, hop, nullptr); });
 }
 
 int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble* ens,
@@ -298,25 +401,35 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
     dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
     const int k = de.k;
     const int64_t p = o.p, n = de.n;
-    MDA_REQUIRE(k <= 64, "EnKF supports k <= 64 in this build");
+    MDA_REQUIRE(k <= 128, "EnKF supports k <= 128 in this build");
+    hipStream_t s = c->stream;
     const double* dZ = to_device(c, "enkf_Z", Z, (size_t)p * k, z_on_device != 0);
     // all inflation methods are multiplicative sqrt(rho) (EnKF.hpp:316-331)
     const double scale = std::sqrt(inflation);
     double* yp = c->get<double>("yp_om", (size_t)p * k);
     double* dd = c->get<double>("d", p);
-    launch_obs_stats(c->stream, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp, dd,
-                     nullptr);
+    int* st = c->get<int>("g_status", 1);
+    launch_zero(s, st, sizeof(int));
+    launch_check_obs(s, o, st);
+    launch_obs_stats(s, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, nullptr, yp, dd,
+                     nullptr, nullptr, nullptr, nullptr, st);
     const int nb = 256;
     double* part = c->get<double>("gg_part", (size_t)nb * k * (k + 1));
     double* EB = c->get<double>("gg_EB", (size_t)k * (k + 1));
     double* F = c->get<double>("gg_F", (size_t)k * k);
-    launch_global_gram(c->stream, yp, dd, o.var, nullptr, k, p, 0, part, nb, EB);
-    launch_global_gram(c->stream, yp, dd, o.var, dZ, k, p, 1, part, nb, F);
+    launch_global_gram(s, yp, dd, o.var, nullptr, k, p, 0, part, nb, EB);
+    launch_global_gram(s, yp, dd, o.var, dZ, k, p, 1, part, nb, F);
     double* T = c->get<double>("gT", (size_t)k * k);
     double* Minv = c->get<double>("gMinv", (size_t)k * k);
     double* scr = c->get<double>("g_scr", (size_t)2 * k * k);
-    int* st = c->get<int>("g_status", 1);
-    launch_enkf_solve_ws(c->stream, k, EB, F, T, Minv, scr, st);
+    launch_enkf_solve_ws(s, k, EB, F, T, Minv, scr, st);
+    {
+      // a failed factorisation or non-finite input ends the call before any
+      // diagnostics or update
+      int hs0 = 0;
+      d2h(c, &hs0, st, sizeof(int));
+      throw_status(hs0 & MDA_ST_SKIP_UPDATE, "EnKF");
+    }
     mda_enkf_diag dg{};
     dg.ensemble_size = k;
     dg.observation_count = p;
@@ -325,61 +438,39 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
     dg.analysis_increment_norm = 0.0;
     double* part2 = c->get<double>("sumsq_part", 1024);
     if (diag) {
-      launch_dev_sumsq(c->stream, de.X, k, n, scale, part2, 512);
+      launch_dev_sumsq(s, de.X, k, n, scale, part2, 512);
       dg.background_spread = std::sqrt(host_sum(c, part2, 512) / ((double)n * k));
-      launch_sumsq(c->stream, dd, p, part2, 512);
+      launch_sumsq(s, dd, p, part2, 512);
       dg.innovation_norm = std::sqrt(host_sum(c, part2, 512));
-      // max / min Kalman gain: n*p*k work, skipped (NaN) beyond 1e12
-      if ((double)n * p * k <= 1e12) {
+      // max / min Kalman gain over the n x p entries of K (EnKF.hpp:202-203);
+      // O(n p k) work, as in the reference
+      {
         double* G = c->get<double>("enkf_G", (size_t)p * k);
         double* mm = c->get<double>("enkf_mm", 2 * 512);
-        launch_enkf_gain_minmax_ws(c->stream, de.X, k, n, scale, Minv, yp, o.var, p, G, mm, 512);
+        launch_enkf_gain_minmax_ws(s, de.X, k, n, scale, Minv, yp, o.var, p, G, mm, 512);
         std::vector<double> hm(2 * 512);
         d2h(c, hm.data(), mm, sizeof(double) * 1024);
-        MDA_HIP(hipStreamSynchronize(c->stream));
+        MDA_HIP(hipStreamSynchronize(s));
         double mx = -std::numeric_limits<double>::infinity(), mn = -mx;
         for (int b = 0; b < 512; ++b) { mx = std::max(mx, hm[2 * b]); mn = std::min(mn, hm[2 * b + 1]); }
         dg.max_kalman_gain = mx;
         dg.min_kalman_gain = mn;
-      } else {
-        dg.max_kalman_gain = dg.min_kalman_gain = std::numeric_limits<double>::quiet_NaN();
       }
-      // cond(S) (EnKF.hpp:205-209)
-      double* cnd = c->get<double>("enkf_cond", 1);
-      if (p <= 128) {
-        double* S = c->get<double>("enkf_S", (size_t)130 * 130);
-        launch_cond_small(c->stream, yp, o.var, k, (int)p, S, cnd);
-        d2h(c, &dg.condition_number, cnd, sizeof(double));
-        MDA_HIP(hipStreamSynchronize(c->stream));
-      } else {
-        std::vector<double> hv(p);
-        d2h(c, hv.data(), o.var, sizeof(double) * p);
-        MDA_HIP(hipStreamSynchronize(c->stream));
-        bool constant = std::all_of(hv.begin(), hv.end(), [&](double v) { return v == hv[0]; });
-        if (constant) {
-          // R = s^2 I and p > k: eig(S) = s^2 (1 + mu(E)/(k-1)) and s^2 (rank deficiency)
-          double* S = c->get<double>("enkf_S", (size_t)130 * 130);
-          launch_sym_maxeig(c->stream, EB, k, S, cnd);
-          double mu = 0.0;
-          d2h(c, &mu, cnd, sizeof(double));
-          MDA_HIP(hipStreamSynchronize(c->stream));
-          dg.condition_number = 1.0 + mu / (double)(k - 1);
-        } else {
-          dg.condition_number = std::numeric_limits<double>::quiet_NaN();
-        }
-      }
+      // cond(S) = ratio of the extreme singular values of the SPD
+      // S = Y'Y'^T/(k-1) + R (EnKF.hpp:205-209)
+      dg.condition_number = enkf_condition_number(c, yp, o.var, k, p, EB);
     }
-    launch_transform_global(c->stream, de.X, k, n, scale, T);
-    int hs = 0;
-    d2h(c, &hs, st, sizeof(int));
+    launch_transform_global(s, de.X, k, n, scale, T, st);
     if (diag) {
-      launch_dev_sumsq(c->stream, de.X, k, n, 1.0, part2, 512);
+      launch_dev_sumsq(s, de.X, k, n, 1.0, part2, 512);
       dg.analysis_spread = std::sqrt(host_sum(c, part2, 512) / ((double)n * k));
     }
-    MDA_HIP(hipStreamSynchronize(c->stream));
-    if (hs) throw mda_error(MDA_E_NUMERIC, "EnKF: (k-1)I + Y'^T R^-1 Y' is not positive definite");
+    int hs = 0;
+    d2h(c, &hs, st, sizeof(int));
+    MDA_HIP(hipStreamSynchronize(s));
     unstage_ensemble(c, ens, de);
     if (diag) *diag = dg;
+    throw_status(hs, "EnKF");
   });
 }
 
@@ -471,10 +562,14 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
                              const std::string& pre, bool queue) {
   check_letkf_params(prm, geom);
   letkf_plan P;
+  // argument blocks with zero padding: the batch's graph key holds their bytes
+  std::memset(&P.ga, 0, sizeof(P.ga));
+  std::memset(&P.sa, 0, sizeof(P.sa));
+  std::memset(&P.bg, 0, sizeof(P.bg));
   P.prm = prm; P.ens = ens; P.geom = geom;
   P.de = stage_ensemble(c, ens);
-  P.o = stage_obs(c, obs, true);
-  P.h = stage_hop(c, hop, P.de.k, P.o.p, P.de.n);
+  P.o = stage_obs(c, obs, true, /*allow_empty=*/true);
+  if (P.o.p > 0) P.h = stage_hop(c, hop, P.de.k, P.o.p, P.de.n);
   const int k = P.k = P.de.k;
   const int64_t p = P.p = P.o.p;
   P.n = P.de.n; P.nx = ens->nx; P.ny = ens->ny; P.nlev = ens->nlev;
@@ -573,6 +668,11 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   ga.crec = c->get<int>(pre + "col_rec", (size_t)ncol * LETKF_CREC_INTS);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
+  if (geom->coord == MDA_COORD_GEOGRAPHIC) {
+    ga.shell_cap = std::max<int64_t>(65536, 4 * nck);
+    ga.shell = c->get<int4>(pre + "shell_list", (size_t)ga.shell_cap);
+    ga.shell_count = c->get<uint32_t>(pre + "shell_count", 1);
+  }
   P.queue = queue;
   if (queue) {
     P.ckeys = c->get<uint32_t>(pre + "cq_keys", ncol);
@@ -627,6 +727,10 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
 // its rows are moved into bucket order afterwards.  Both idempotent.
 static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
   letkf_gram_args& ga = P.ga;
+  // the analysis's status word, then the observations' validity
+  launch_zero(s, P.sa.status, sizeof(int));
+  launch_check_obs(s, P.o, P.sa.status);
+  if (P.p == 0) return;  // every column takes the empty-column branch
   launch_bucket_keys(s, P.rcoord, P.ccoord, P.p, P.bg, P.keys, P.vals);
   radix_sort_pairs(s, P.keys, P.vals, P.keys2, P.vals2, P.p, P.key_bits, P.hist, P.hist_cap);
   launch_bucket_ranges(s, P.keys, P.p, P.nbuckets, P.bstart);
@@ -641,10 +745,12 @@ static void enqueue_buckets(hipStream_t s, letkf_plan& P) {
 
 static void enqueue_local(hipStream_t s, letkf_plan& P) {
   letkf_gram_args& ga = P.ga;
+  if (P.p == 0) return;
+  int* st = P.sa.status;
   float* y32 = P.prm->precision != MDA_PREC_FP64 ? const_cast<float*>(ga.y32) : nullptr;
   if (P.h.hx) {
     launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, P.h.hx, P.o.yo,
-                     nullptr, P.yp, P.dd, nullptr);
+                     nullptr, P.yp, P.dd, nullptr, nullptr, nullptr, nullptr, st);
     launch_gather_rows(s, P.vals, P.p, P.k, P.yp, P.dd, const_cast<double*>(ga.yp),
                        const_cast<double*>(ga.d), y32);
   } else if (P.tkeys) {
@@ -656,11 +762,11 @@ static void enqueue_local(hipStream_t s, letkf_plan& P) {
     launch_invert_order(s, P.vals, P.p, P.ipos);
     launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, nullptr, P.o.yo,
                      nullptr, const_cast<double*>(ga.yp), const_cast<double*>(ga.d), nullptr,
-                     P.tvals, y32, P.ipos);
+                     P.tvals, y32, P.ipos, st);
   } else {
     launch_obs_stats(s, P.de.X, P.k, P.n, P.p, P.h.row_ptr, P.h.col, P.h.w, nullptr, P.o.yo,
                      nullptr, const_cast<double*>(ga.yp), const_cast<double*>(ga.d), nullptr,
-                     P.vals, y32);
+                     P.vals, y32, nullptr, st);
   }
 }
 
@@ -707,7 +813,9 @@ static void enqueue_gram(hipStream_t s, letkf_plan& P, int64_t c0, int64_t nc, i
     ga.order = P.ccols;  // the sort leaves the ordered pairs in the first arrays
     ga.queue = P.qcount;
   }
+  if (ga.shell_count) launch_zero(s, ga.shell_count, sizeof(uint32_t));
   launch_letkf_gram(s, ga);
+  launch_letkf_shell_fixup(s, ga, P.sa.status);
 }
 
 // Event marks of one pass: gram start, gram end, E1, E2, E3 (tridiagonal
@@ -720,20 +828,20 @@ struct pass_marks {
 // receives one set of events per pass.
 static void enqueue_columns(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg,
                             const std::vector<pass_marks>* marks) {
-  launch_zero(s, P.sa.status, sizeof(int));
   launch_zero(s, P.sweeps, sizeof(int) * P.ncol);
+  if (P.p == 0) launch_zero(s, P.ga.nloc, sizeof(int) * P.ncol);
   int pass = 0;
   for (int64_t c0 = 0; c0 < P.ncol; c0 += P.chunk, ++pass) {
     const int64_t nc = std::min(P.chunk, P.ncol - c0);
     const pass_marks* m = marks ? &(*marks)[pass] : nullptr;
     if (m) MDA_HIP(hipEventRecord(m->e[0], s));
-    enqueue_gram(s, P, c0, nc, cfg.first, cfg.second != 0);
+    if (P.p > 0) enqueue_gram(s, P, c0, nc, cfg.first, cfg.second != 0);
     if (m) MDA_HIP(hipEventRecord(m->e[1], s));
     letkf_solve_args sa = solve_pass(P, c0, nc);
     for (int i = 0; i < 3; ++i) sa.ev_stage[i] = m ? m->e[2 + i] : nullptr;
     launch_letkf_solve(s, sa);
     if (m) MDA_HIP(hipEventRecord(m->e[5], s));
-    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, c0, nc);
+    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, P.sa.status, c0, nc);
     if (m) MDA_HIP(hipEventRecord(m->e[6], s));
   }
 }
@@ -756,8 +864,9 @@ static void finish_letkf(mda_ctx* c, letkf_plan& P, mda_letkf_diag* diag) {
   c->last_T = P.chunk == P.ncol ? P.sa.T : nullptr;
   c->last_k = P.k; c->last_ncol = P.ncol; c->last_nloc = P.ga.nloc;
   c->last_flag = P.eig_path ? P.sa.flag : nullptr;
-  if (hstat) throw mda_error(MDA_E_NUMERIC, "LETKF: a local Pa is not positive definite");
+  throw_status(hstat & MDA_ST_SKIP_UPDATE, "LETKF");
   unstage_ensemble(c, P.ens, P.de);
+  throw_status(hstat, "LETKF");
   if (diag) {
     diag->n_columns = P.ncol;
     diag->total_local_obs = hs[0];
@@ -858,6 +967,41 @@ static std::pair<int, int> autotune_gram(mda_ctx* c, letkf_plan& P, bool allow_q
   return best;
 }
 
+// Bytes of everything an enqueued analysis's kernels read from the plan
+// (see mda_letkf_analyse_batch).  The argument blocks are value-initialised
+// (padding zero) before their fields are set, so their raw bytes are
+// deterministic.
+extern "C++" {
+template <class T>
+static void sig_add(std::string& out, const T& v) {
+  out.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+}
+
+static std::string plan_signature(const letkf_plan& P) {
+  std::string sg;
+  sig_add(sg, P.ga);
+  sig_add(sg, P.sa);
+  sig_add(sg, *P.prm);
+  for (const void* ptr : {(const void*)P.de.X, (const void*)P.o.yo, (const void*)P.o.var,
+                          (const void*)P.o.a, (const void*)P.o.b, (const void*)P.o.c,
+                          (const void*)P.h.row_ptr, (const void*)P.h.col, (const void*)P.h.w,
+                          (const void*)P.h.hx, (const void*)P.rcoord, (const void*)P.ccoord,
+                          (const void*)P.yp, (const void*)P.dd, (const void*)P.keys,
+                          (const void*)P.vals, (const void*)P.keys2, (const void*)P.vals2,
+                          (const void*)P.hist, (const void*)P.tkeys, (const void*)P.tvals,
+                          (const void*)P.ipos, (const void*)P.bstart, (const void*)P.sweeps,
+                          (const void*)P.stats, (const void*)P.ckeys, (const void*)P.ccols,
+                          (const void*)P.ckeys2, (const void*)P.ccols2, (const void*)P.chist,
+                          (const void*)P.qcount})
+    sig_add(sg, ptr);
+  for (int64_t v : {(int64_t)P.k, P.p, P.n, P.nx, P.ny, P.nlev, P.ncol, P.chunk,
+                    (int64_t)P.nbuckets, (int64_t)P.key_bits, P.hist_cap, (int64_t)P.tap_bits,
+                    (int64_t)P.geo, (int64_t)P.eig_path, (int64_t)P.queue, P.chist_cap})
+    sig_add(sg, v);
+  return sg;
+}
+
 static void enqueue_plan(hipStream_t s, letkf_plan& P, std::pair<int, int> cfg) {
   enqueue_buckets(s, P);
   enqueue_local(s, P);
@@ -917,19 +1061,16 @@ int mda_letkf_analyse_batch(mda_ctx* c, int32_t nb, const mda_letkf_params* prm,
     int replayed = 0;
     MDA_HIP(hipStreamSynchronize(s));
     if (use_graph) {
+      // The graph bakes in every kernel argument, so its key holds every
+      // device pointer and every value derived from the inputs that the
+      // enqueued kernels take: the plan's argument blocks of the Gram and
+      // the solve (pointers into the workspace and the caller's arrays,
+      // bucket grid, radius, forms), its own buffers and sizes, the caller's
+      // parameters and the Gram launch shape.
       std::string sig = std::to_string(nb);
-      for (int i = 0; i < nb; ++i) {
-        const letkf_plan& P = plans[i];
-        char buf[512];
-        snprintf(buf, sizeof(buf), "|%p,%p,%p,%p,%p,%p,%d,%lld,%lld,%lld,%d,%d,%d,%d,%d,%d,%.17g,%.17g,%p,%d,%d",
-                 (const void*)P.de.X, (const void*)P.o.yo, (const void*)P.ga.C, (const void*)P.sa.T,
-                 (const void*)P.h.row_ptr, (const void*)P.h.hx, P.k, (long long)P.p,
-                 (long long)P.ncol, (long long)P.nlev, P.prm->precision, P.prm->localization,
-                 P.prm->rweight, P.prm->inflation_form, P.prm->sqrt_form, P.prm->combine,
-                 P.prm->inflation, P.prm->radius, (const void*)P.sa.rho_col, cfg[i].first,
-                 cfg[i].second);
-        sig += buf;
-      }
+      for (int i = 0; i < nb; ++i)
+        sig += "|" + plan_signature(plans[i]) + std::to_string(cfg[i].first) + "," +
+               std::to_string(cfg[i].second);
       auto it = c->graphs.find(sig);
       if (it == c->graphs.end()) {
         hipGraph_t graph = nullptr;
@@ -1077,6 +1218,7 @@ int mda_letkf_inflation_features(mda_ctx* c, const mda_letkf_params* prm,
     fp.rho_column = nullptr;
     letkf_plan P = plan_letkf(c, &fp, const_cast<mda_ensemble*>(ens), geom, obs, hop, "",
                               false);
+    MDA_REQUIRE(P.p > 0, "inflation features need observations");
     enqueue_buckets(s, P);
     enqueue_local(s, P);
     const int64_t ncol = P.ncol;

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_obs_stats(
     const double* __restrict__ yo, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ opos, double* __restrict__ ybar,
     double* __restrict__ yp_om, double* __restrict__ d, double* __restrict__ yp_mm,
-    float* __restrict__ y32) {
+    float* __restrict__ y32, int* __restrict__ status) {
   extern __shared__ double tile[];  // [OBS_TILE][k + 1] (padded rows)
   const int kp = k + 1;
   __shared__ double mean_s[OBS_TILE];
@@ -181,8 +181,11 @@ __global__ __launch_bounds__(256) void k_obs_stats(
     const int64_t o = order ? (int64_t)order[o0 + ob] : o0 + ob;
     const int64_t q = opos ? (int64_t)opos[o] : o0 + ob;  // output position
     pos_s[ob] = q;
+    const double dv = yo[o] - m;
     if (ybar) ybar[q] = m;
-    if (d) d[q] = yo[o] - m;
+    if (d) d[q] = dv;
+    // a non-finite H(x) (hence mean, Y' row and innovation) or observation
+    if (status && !isfinite(dv)) atomicOr(status, MDA_ST_NONFINITE_OBS);
   }
   __syncthreads();
   for (int e = tid; e < nob * k; e += nt) {
@@ -206,14 +209,14 @@ void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t 
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
                       double* d, double* yp_mm, const uint32_t* order, float* y32,
-                      const uint32_t* opos) {
+                      const uint32_t* opos, int* status) {
   if (p <= 0) return;
   const int64_t tiles = (p + OBS_TILE - 1) / OBS_TILE;
   MDA_REQUIRE(tiles < ((int64_t)1 << 31) - 8, "too many observations");
   const unsigned nb = (unsigned)((tiles + 7) / 8 * 8);
   size_t shm = sizeof(double) * OBS_TILE * (k + 1);
   hipLaunchKernelGGL(k_obs_stats, dim3(nb), dim3(256), shm, s, X, k, n, p, row_ptr, col, w,
-                     hx_mm, yo, order, opos, ybar, yp_om, d, yp_mm, y32);
+                     hx_mm, yo, order, opos, ybar, yp_om, d, yp_mm, y32, status);
   MDA_LAUNCH_CHECK();
 }
 
@@ -266,26 +269,27 @@ void launch_invert_order(hipStream_t s, const uint32_t* order, int64_t p, uint32
 //   mode 1: G[a][j] = sum_o Y'[o][a] D[o][j] / var_o with the perturbed
 //           innovation D[o][j] = d_o + sqrt(var_o) Z[o][j] - Y'[o][j]
 //           (EnKF.hpp:212-224; Z is [p][k]).
-// Fixed grid, per-block partials, then a fixed-order reduction.
+// Fixed grid, per-block partials, then a fixed-order reduction.  Blocks of
+// 256 threads take 32 observations at a time up to k (k+1) = 5120 entries;
+// beyond (64 < k <= 128) blocks of 1024 threads take 16.
 // ---------------------------------------------------------------------------
-constexpr int GG_CHUNK = 32;
+constexpr int GG_ACC = 20;  // entries per thread
 
-__global__ __launch_bounds__(256) void k_global_gram_partial(
+__global__ __launch_bounds__(1024) void k_global_gram_partial(
     const double* __restrict__ yp_om, const double* __restrict__ d,
     const double* __restrict__ var, const double* __restrict__ Z, int k, int64_t p, int mode,
-    double* __restrict__ partial) {
+    int chunk, double* __restrict__ partial) {
   extern __shared__ double sh[];
   const int kq = mode == 0 ? k + 1 : k;
-  double* ys = sh;                     // [GG_CHUNK][k]
-  double* qs = ys + GG_CHUNK * k;      // [GG_CHUNK][kq]
-  double* rs = qs + GG_CHUNK * kq;     // [GG_CHUNK]
+  double* ys = sh;                  // [chunk][k]
+  double* qs = ys + chunk * k;      // [chunk][kq]
+  double* rs = qs + chunk * kq;     // [chunk]
   const int tid = threadIdx.x, nt = blockDim.x;
   const int ne = k * kq;
-  double acc[20];
-  for (int t = 0; t < 20; ++t) acc[t] = 0.0;
-  for (int64_t o0 = (int64_t)blockIdx.x * GG_CHUNK; o0 < p;
-       o0 += (int64_t)gridDim.x * GG_CHUNK) {
-    int nob = (int)((p - o0) < GG_CHUNK ? (p - o0) : GG_CHUNK);
+  double acc[GG_ACC];
+  for (int t = 0; t < GG_ACC; ++t) acc[t] = 0.0;
+  for (int64_t o0 = (int64_t)blockIdx.x * chunk; o0 < p; o0 += (int64_t)gridDim.x * chunk) {
+    int nob = (int)((p - o0) < chunk ? (p - o0) : chunk);
     __syncthreads();
     for (int e = tid; e < nob * k; e += nt) ys[e] = yp_om[(size_t)o0 * k + e];
     for (int ob = tid; ob < nob; ob += nt) rs[ob] = 1.0 / var[o0 + ob];
@@ -296,8 +300,8 @@ __global__ __launch_bounds__(256) void k_global_gram_partial(
       double v;
       if (mode == 0)
         v = (c < k) ? ys[ob * k + c] : d[o];
-      else
-        v = d[o] + sqrt(var[o]) * Z[(size_t)o * k + c] - ys[ob * k + c];
+      else  // an invalid observation (var = +inf, weight 0) contributes nothing
+        v = isinf(var[o]) ? 0.0 : d[o] + sqrt(var[o]) * Z[(size_t)o * k + c] - ys[ob * k + c];
       qs[e] = v;
     }
     __syncthreads();
@@ -326,10 +330,12 @@ void launch_global_gram(hipStream_t s, const double* yp_om, const double* d,
                         const double* var, const double* Z, int k, int64_t p, int mode,
                         double* partial, int nblocks, double* out) {
   const int kq = mode == 0 ? k + 1 : k;
-  MDA_REQUIRE((k * kq + 255) / 256 <= 20, "global Gram supports k <= 70");
-  size_t shm = sizeof(double) * (GG_CHUNK * k + GG_CHUNK * kq + GG_CHUNK);
-  hipLaunchKernelGGL(k_global_gram_partial, dim3(nblocks), dim3(256), shm, s, yp_om, d, var, Z,
-                     k, p, mode, partial);
+  const int nt = k * kq <= 256 * GG_ACC ? 256 : 1024;
+  const int chunk = nt == 256 ? 32 : 16;
+  MDA_REQUIRE((k * kq + nt - 1) / nt <= GG_ACC, "global Gram supports k <= 128");
+  size_t shm = sizeof(double) * (chunk * k + chunk * kq + chunk);
+  hipLaunchKernelGGL(k_global_gram_partial, dim3(nblocks), dim3(nt), shm, s, yp_om, d, var, Z,
+                     k, p, mode, chunk, partial);
   MDA_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(k * kq, 256)), dim3(256), 0, s, partial,
                      nblocks, k * kq, out);
@@ -370,7 +376,7 @@ __global__ __launch_bounds__(256) void k_etkf_solve(int k, const double* __restr
     int i = idx / k, j = idx % k;
     T[idx] = wa[i] + sq * M[idx];
   }
-  if (tid == 0) *status = ok ? 0 : 1;
+  if (tid == 0 && !ok) atomicOr(status, MDA_ST_NOT_SPD);
 }
 
 void launch_etkf_solve_ws(hipStream_t s, int k, const double* EB, double* T, double* scratch,
@@ -408,7 +414,7 @@ __global__ __launch_bounds__(256) void k_enkf_solve(int k, const double* __restr
     for (int c = 0; c < k; ++c) s += Minv[i * k + c] * F[c * k + j];
     T[idx] = (i == j ? 1.0 : 0.0) + s;
   }
-  if (tid == 0) *status = ok ? 0 : 1;
+  if (tid == 0 && !ok) atomicOr(status, MDA_ST_NOT_SPD);
 }
 
 void launch_enkf_solve_ws(hipStream_t s, int k, const double* EB, const double* F, double* T,
@@ -421,11 +427,17 @@ void launch_enkf_solve_ws(hipStream_t s, int k, const double* EB, const double* 
 // ---------------------------------------------------------------------------
 // Global transform: X_a[j][i] = xbar_i + sum_m (scale (x_m,i - xbar_i)) T[m][j]
 // (ETKF.hpp:163-169, EnKF.hpp:216-227).  T in LDS; one thread per state index.
+// Nothing is written when an earlier stage of the analysis failed (status bits
+// MDA_ST_SKIP_UPDATE); a non-finite background value sets
+// MDA_ST_NONFINITE_STATE (the analysis at that index is then non-finite, as
+// the reference's arithmetic makes it).
 // ---------------------------------------------------------------------------
 template <int KMAX>
 __global__ __launch_bounds__(256) void k_transform_global(double* __restrict__ X, int k,
                                                           int64_t n, double scale,
-                                                          const double* __restrict__ T) {
+                                                          const double* __restrict__ T,
+                                                          int* __restrict__ status) {
+  if (status && (*status & MDA_ST_SKIP_UPDATE)) return;
   extern __shared__ double Ts[];
   for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) Ts[idx] = T[idx];
   __syncthreads();
@@ -438,6 +450,7 @@ __global__ __launch_bounds__(256) void k_transform_global(double* __restrict__ X
     for (int j = 0; j < KMAX; ++j)
       if (j < k) { xp[j] = X[(size_t)j * n + i]; s += xp[j]; }
     double m = s * rk;
+    if (status && !isfinite(m)) atomicOr(status, MDA_ST_NONFINITE_STATE);
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
       if (j < k) xp[j] = (xp[j] - m) * scale;
@@ -452,15 +465,15 @@ __global__ __launch_bounds__(256) void k_transform_global(double* __restrict__ X
 }
 
 void launch_transform_global(hipStream_t s, double* X, int k, int64_t n, double scale,
-                             const double* T) {
+                             const double* T, int* status) {
   size_t shm = sizeof(double) * k * k;
   unsigned g = grid_for(n, 256, 16384);
   if (k <= 32)
-    hipLaunchKernelGGL(k_transform_global<32>, dim3(g), dim3(256), shm, s, X, k, n, scale, T);
+    hipLaunchKernelGGL(k_transform_global<32>, dim3(g), dim3(256), shm, s, X, k, n, scale, T, status);
   else if (k <= 64)
-    hipLaunchKernelGGL(k_transform_global<64>, dim3(g), dim3(256), shm, s, X, k, n, scale, T);
+    hipLaunchKernelGGL(k_transform_global<64>, dim3(g), dim3(256), shm, s, X, k, n, scale, T, status);
   else if (k <= 128)
-    hipLaunchKernelGGL(k_transform_global<128>, dim3(g), dim3(256), shm, s, X, k, n, scale, T);
+    hipLaunchKernelGGL(k_transform_global<128>, dim3(g), dim3(256), shm, s, X, k, n, scale, T, status);
   else
     throw mda_error(MDA_E_UNSUPPORTED, "global transform supports k <= 128");
   MDA_LAUNCH_CHECK();

@@ -7,6 +7,7 @@
 //   3. per-column k x k solve: parallel Jacobi (symmetric square root) or
 //      Cholesky (reference square root) -> transform T_g;
 //   4. X_a = xbar + X' T_g for every level of the column.
+#include "mda_crmath.hpp"
 #include "mda_device.hpp"
 #include "mda_internal.hpp"
 
@@ -454,6 +455,16 @@ __device__ inline row_ranges row_from_table(const letkf_gram_args& a, int row0, 
   return rr;
 }
 
+// A GEOGRAPHIC candidate inside the shell around the membership threshold
+// (shell_width): (column, observation, the chord test's decision) goes to
+// the pass's boundary list, and k_letkf_shell_fixup decides it exactly after
+// the Gram.  Rare: about one pair per analysis at 1e6 observations.
+__device__ inline void shell_push(const letkf_gram_args& a, int64_t g, int64_t t, bool in) {
+  const unsigned slot = atomicAdd(a.shell_count, 1u);
+  if (slot < (unsigned)a.shell_cap)
+    a.shell[slot] = make_int4((int)g, (int)(t & 0xffffffff), (int)(t >> 32), in ? 1 : 0);
+}
+
 // Column iteration of the Gram kernels: a grid-stride walk, or (work-queue
 // mode) lane 0 of each wave takes the next position of a cost-sorted order
 // with a global atomic counter; every wave exits once the counter passes ncol.
@@ -512,6 +523,7 @@ __device__ __forceinline__ void gram_f64_body(const letkf_gram_args& a) {
   // haversine test d <= r on unit vectors
   const double half_ang = a.radius / (2.0 * kEarthRadiusKm);
   const double chord2_r = half_ang >= kPi / 2.0 ? 5.0 : 4.0 * sin(half_ang) * sin(half_ang);
+  const double shellw = shell_width(chord2_r);
   const int64_t nitems = a.ncol * NPART;
   int64_t it = a.queue ? queue_next(a.queue) : (int64_t)blockIdx.x * 4 + wv;
   for (; it < nitems; it = a.queue ? queue_next(a.queue) : it + (int64_t)gridDim.x * 4) {
@@ -551,15 +563,12 @@ __device__ __forceinline__ void gram_f64_body(const letkf_gram_args& a) {
               const double dx = rx - a.ux[t], dy = ry - a.uy[t], dz = rz - a.uz[t];
               const double c2 = dx * dx + dy * dy + dz * dz;
               in = c2 <= chord2_r;
+              if (fabs(c2 - chord2_r) <= shellw && part == 0) shell_push(a, g, t, in);
               dist = 2.0 * kEarthRadiusKm * asin(fmin(1.0, 0.5 * sqrt(c2)));
             } else {
               const double dx = rx - a.oa[t], dy = ry - a.ob[t];
-              double d2 = dx * dx + dy * dy;
-              if (a.coord == MDA_COORD_CARTESIAN) {
-                const double dz = rz - a.oc[t];
-                d2 += dz * dz;
-              }
-              dist = sqrt(d2);
+              const double dz = a.coord == MDA_COORD_CARTESIAN ? rz - a.oc[t] : 0.0;
+              dist = sqrt(dist2_exact(dx, dy, dz));
               in = dist <= a.radius;
             }
             if (in)
@@ -787,6 +796,7 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
   // chord threshold for GEOGRAPHIC: |u - v|^2 <= 4 sin^2(r / 2R)
   const double half_ang = a.radius / (2.0 * kEarthRadiusKm);
   const double chord2_r = half_ang >= kPi / 2.0 ? 5.0 : 4.0 * sin(half_ang) * sin(half_ang);
+  const double shellw = shell_width(chord2_r);
   // XCD-contiguous virtual block index (gridDim.x is a multiple of 8)
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
@@ -994,14 +1004,12 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
               const double dx = rx - ca, dy = ry - cb, dz = rz - cc;
               const double c2 = dx * dx + dy * dy + dz * dz;
               in = c2 <= chord2_r;
+              if (fabs(c2 - chord2_r) <= shellw) shell_push(a, g, t, in);
               dist = 2.f * (float)kEarthRadiusKm * asinf(fminf(1.f, 0.5f * sqrtf((float)c2)));
             } else {
               const double dx = rx - ca, dy = ry - cb;
-              double d2 = dx * dx + dy * dy;
-              if (a.coord == MDA_COORD_CARTESIAN) {
-                const double dz = rz - cc;
-                d2 += dz * dz;
-              }
+              const double dz = a.coord == MDA_COORD_CARTESIAN ? rz - cc : 0.0;
+              const double d2 = dist2_exact(dx, dy, dz);
               in = sqrt(d2) <= a.radius;
               dist = (float)sqrt(d2);
             }
@@ -1156,6 +1164,49 @@ void k_letkf_gram_mixed_w(letkf_gram_args a) {
 }
 
 // ---------------------------------------------------------------------------
+// Exact decisions for the boundary list of one pass (shell_push): each listed
+// (column, observation) pair is decided by the haversine distance of the
+// reference (Location.hpp:349-357) with correctly rounded sin / cos / atan2.
+// Where that differs from the chord test the Gram used, the column's local
+// count moves by one and its C and b take (+ or -) the pair's weighted
+// contribution w y'y'^T and w y' d (w = GC(d/c) / var, which this close to
+// the support is below 1e-27 / var, or 1 / var with the hard cutoff).  One
+// wavefront per pair; a list longer than its capacity sets
+// MDA_ST_BOUNDARY_OVERFLOW.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_letkf_shell_fixup(letkf_gram_args a,
+                                                           int* __restrict__ status) {
+  const unsigned cnt = *a.shell_count;
+  if (cnt > (unsigned)a.shell_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(status, MDA_ST_BOUNDARY_OVERFLOW);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int k = a.k;
+  for (int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); e < (int64_t)cnt;
+       e += (int64_t)gridDim.x * 4) {
+    const int4 v = a.shell[e];
+    const int64_t g = v.x;
+    const int64_t t = (int64_t)(uint32_t)v.y | ((int64_t)v.z << 32);
+    const bool chord_in = v.w != 0;
+    const double dist = cr::haversine_km_cr(a.col_a[g], a.col_b[g], a.oa[t], a.ob[t]);
+    const bool exact_in = dist <= a.radius;
+    if (exact_in == chord_in) continue;
+    if (lane == 0) atomicAdd(&a.nloc[g], exact_in ? 1 : -1);
+    const double w =
+        (a.localization == MDA_LOC_GASPARI_COHN ? gaspari_cohn(dist / (a.radius / 2.0)) : 1.0) *
+        a.rinv[t];
+    const double sg = exact_in ? w : -w;
+    if (sg == 0.0) continue;
+    const double* y = a.yp + (size_t)t * k;
+    double* Cg = a.C + (size_t)g * k * k;
+    for (int idx = lane; idx < k * k; idx += 64)
+      atomicAdd(&Cg[idx], sg * y[idx / k] * y[idx % k]);
+    for (int i = lane; i < k; i += 64) atomicAdd(&a.b[(size_t)g * k + i], sg * y[i] * a.d[t]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Local innovation and spread statistics of each column, the inputs of the
 // learned inflation.  Same candidate search and weights as the FP64 Gram
 // (w_o = localisation x 1/var_o); per column
@@ -1202,14 +1253,13 @@ __global__ __launch_bounds__(256) void k_letkf_features(letkf_gram_args a,
           double dist;
           if (a.coord == MDA_COORD_GEOGRAPHIC) {
             dist = haversine_km(cw.ca, cw.cb, a.oa[t], a.ob[t]);
+            // near the radius, the reference's distance to the last bit
+            if (fabs(dist - a.radius) <= 1e-9 * a.radius)
+              dist = cr::haversine_km_cr(cw.ca, cw.cb, a.oa[t], a.ob[t]);
           } else {
             const double dx = cw.ca - a.oa[t], dy = cw.cb - a.ob[t];
-            double d2 = dx * dx + dy * dy;
-            if (a.coord == MDA_COORD_CARTESIAN) {
-              const double dz = cw.cc - a.oc[t];
-              d2 += dz * dz;
-            }
-            dist = sqrt(d2);
+            const double dz = a.coord == MDA_COORD_CARTESIAN ? cw.cc - a.oc[t] : 0.0;
+            dist = sqrt(dist2_exact(dx, dy, dz));
           }
           if (dist <= a.radius) {
             ++nl;
@@ -1276,6 +1326,13 @@ void launch_obs_spread(hipStream_t s, const double* s_yp, int64_t p, int k, doub
   if (p <= 0) return;
   hipLaunchKernelGGL(k_obs_spread, dim3(grid_for(p, 256, 65536)), dim3(256), 0, s, s_yp, p, k,
                      sp);
+  MDA_LAUNCH_CHECK();
+}
+
+void launch_letkf_shell_fixup(hipStream_t s, const letkf_gram_args& a, int* status) {
+  if (a.coord != MDA_COORD_GEOGRAPHIC) return;
+  MDA_REQUIRE(a.shell && a.shell_count, "boundary list missing");
+  hipLaunchKernelGGL(k_letkf_shell_fixup, dim3(64), dim3(256), 0, s, a, status);
   MDA_LAUNCH_CHECK();
 }
 
@@ -1530,7 +1587,9 @@ constexpr int TR_C = 16, TR_L = 8;
 __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X, int k,
                                                          int64_t ncol, int64_t nlev,
                                                          const double* __restrict__ T,
-                                                         int64_t col0, int64_t nc) {
+                                                         int64_t col0, int64_t nc,
+                                                         int* __restrict__ status) {
+  if (*status & MDA_ST_SKIP_UPDATE) return;
   extern __shared__ double xs[];  // [TR_L][TR_C][k+1]
   __shared__ double xm[TR_L][TR_C];
   const int tid = threadIdx.x, nt = blockDim.x, ld = k + 1;
@@ -1561,6 +1620,7 @@ __global__ __launch_bounds__(256) void k_letkf_transform(double* __restrict__ X,
       double s = 0.0;
       for (int m = 0; m < k; ++m) s += row[m];
       double mean = s * rk;
+      if (!isfinite(mean)) atomicOr(status, MDA_ST_NONFINITE_STATE);
       xm[l][c] = mean;
       for (int m = 0; m < k; ++m) row[m] -= mean;
     }
@@ -1622,7 +1682,9 @@ template <int K>
 __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict__ X,
                                                              int64_t ncol, int64_t nlev,
                                                              const double* __restrict__ T,
-                                                             int64_t col0, int64_t nc) {
+                                                             int64_t col0, int64_t nc,
+                                                             int* __restrict__ status) {
+  if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int KP = K + 2;  // row pad: 16-byte aligned, banks shift by 4 per row
   extern __shared__ double xs[];  // [XT_L][XT_C][KP]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1668,7 +1730,9 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
             s1 += row[m + 1 < K ? m + 1 : m];
           }
         }
-        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = (s0 + s1) * rk * cmean + (a0 + a1);
+        const double xa = (s0 + s1) * rk * cmean + (a0 + a1);
+        if (!isfinite(xa)) atomicOr(status, MDA_ST_NONFINITE_STATE);
+        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = xa;
       }
     }
   }
@@ -1690,8 +1754,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
                                                               int64_t nlev,
                                                               const double* __restrict__ T,
-                                                              int64_t col0, int64_t nc) {
+                                                              int64_t col0, int64_t nc,
+                                                              int* __restrict__ status) {
   static_assert(K % 16 == 0 && K <= 64, "K must be a multiple of 16, at most 64");
+  if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int NTJ = K / 16, NS = K / 4;
   __shared__ double xs[XM_C * K * XM_L];  // [column][member or j][level]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1768,6 +1834,14 @@ void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
         acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tb[s4][tj], acc[tj], 0, 0, 0);
     }
     __syncthreads();  // every wavefront has read its X
+    {
+      bool fin = true;
+#pragma unroll
+      for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) fin = fin && isfinite(acc[tj][r]);
+      if (__ballot(!fin) != 0ull && lane == 0) atomicOr(status, MDA_ST_NONFINITE_STATE);
+    }
     // D[kk + 4r][i16] of tile tj = X_a[level kk + 4r][j = 16 tj + i16] -> xs[c][j][l]
 #pragma unroll
     for (int tj = 0; tj < NTJ; ++tj)
@@ -1789,7 +1863,8 @@ bool letkf_transform_reg_supported(int k) {
 }
 
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T, int64_t col0, int64_t nc) {
+                            const double* T, int* status, int64_t col0, int64_t nc) {
+  MDA_REQUIRE(status != nullptr, "transform needs the analysis status word");
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
   if (nc < 0) nc = ncol;
   MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
@@ -1798,10 +1873,10 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
     MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
     switch (k) {
-      case 16: hipLaunchKernelGGL(k_letkf_transform_mfma<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
-      case 32: hipLaunchKernelGGL(k_letkf_transform_mfma<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
-      case 48: hipLaunchKernelGGL(k_letkf_transform_mfma<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
-      default: hipLaunchKernelGGL(k_letkf_transform_mfma<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc); break;
+      case 16: hipLaunchKernelGGL(k_letkf_transform_mfma<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      case 32: hipLaunchKernelGGL(k_letkf_transform_mfma<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      case 48: hipLaunchKernelGGL(k_letkf_transform_mfma<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      default: hipLaunchKernelGGL(k_letkf_transform_mfma<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
     }
     MDA_LAUNCH_CHECK();
     return;
@@ -1813,7 +1888,7 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
 #define MDA_XT(KK)                                                                          \
   case KK:                                                                                 \
     hipLaunchKernelGGL(k_letkf_transform_reg<KK>, dim3(nbr), dim3(256), shr, s, X, ncol, nlev, T, \
-                       col0, nc);                                                            \
+                       col0, nc, status);                                                    \
     break;
     switch (k) { MDA_XT(8) MDA_XT(16) MDA_XT(20) MDA_XT(32) MDA_XT(40) MDA_XT(48) MDA_XT(64) }
 #undef MDA_XT
@@ -1825,7 +1900,7 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
   MDA_REQUIRE(tiles < (int64_t)1 << 31, "transform grid too large");
   const unsigned nb = (unsigned)((tiles + 7) / 8 * 8);
   hipLaunchKernelGGL(k_letkf_transform, dim3(nb), dim3(256), shm, s, X, k, ncol, nlev, T, col0,
-                     nc);
+                     nc, status);
   MDA_LAUNCH_CHECK();
 }
 

@@ -12,14 +12,35 @@ namespace mda {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kEarthRadiusKm = 6371.0;
 
-// Location::haversine (Location.hpp:349-357), same formula and constants.
+// Location::haversine (Location.hpp:349-357), same formula, constants and
+// operation order, with no FMA contraction (the reference's x86-64 build
+// rounds every product and sum separately).
 __device__ inline double haversine_km(double lat1, double lon1, double lat2, double lon2) {
-  double dlat = (lat2 - lat1) * kPi / 180.0;
-  double dlon = (lon2 - lon1) * kPi / 180.0;
-  double s1 = sin(dlat / 2), s2 = sin(dlon / 2);
-  double a = s1 * s1 + cos(lat1 * kPi / 180.0) * cos(lat2 * kPi / 180.0) * s2 * s2;
-  double c = 2 * atan2(sqrt(a), sqrt(1 - a));
+#pragma clang fp contract(off)
+  const double dlat = (lat2 - lat1) * kPi / 180.0;
+  const double dlon = (lon2 - lon1) * kPi / 180.0;
+  const double s1 = sin(dlat / 2), s2 = sin(dlon / 2);
+  const double a = s1 * s1 + cos(lat1 * kPi / 180.0) * cos(lat2 * kPi / 180.0) * s2 * s2;
+  const double c = 2 * atan2(sqrt(a), sqrt(1 - a));
   return kEarthRadiusKm * c;
+}
+
+// Squared GRID / CARTESIAN distance of Location::distance_to
+// (Location.hpp:205-224) in its operation order, (dx dx + dy dy) + dz dz with
+// every product rounded (no FMA), so that a distance equal to the radius in
+// the reference's arithmetic is equal here too.  dz = 0 for GRID.
+__device__ inline double dist2_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Relative half-width of the shell around the GEOGRAPHIC membership
+// threshold chord2_r = 4 sin^2(r / 2R) inside which the chord test on unit
+// vectors (equal to the haversine test in exact arithmetic) is not trusted:
+// |c2 - chord2_r| <= shell_width(chord2_r).  The rounding of c2 (about
+// 1.5e-15 |u - v| absolute) is covered 700 times over.
+__host__ __device__ inline double shell_width(double chord2_r) {
+  return 1e-9 * chord2_r + 1e-12 * sqrt(chord2_r);
 }
 
 // Gaspari & Cohn (1999) eq. (4.10) in z = d / c; zero for z >= 2.

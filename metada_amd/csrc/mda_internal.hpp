@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -34,6 +35,24 @@ struct mda_error : std::runtime_error {
   } while (0)
 
 #define MDA_LAUNCH_CHECK() MDA_HIP(hipGetLastError())
+
+// Bits of an analysis's device status word (zeroed at the start of each
+// analysis; kernels OR into it):
+//   MDA_ST_NOT_SPD          a Cholesky factor did not exist (non-SPD matrix)
+//   MDA_ST_NONFINITE_OBS    a non-finite H(x), observation-space mean or
+//                           innovation
+//   MDA_ST_NONFINITE_STATE  a non-finite background value reached the update
+// The update kernels write nothing when a MDA_ST_SKIP_UPDATE bit is set, so
+// an analysis that fails before its update leaves the ensemble unchanged.
+//   MDA_ST_BOUNDARY_OVERFLOW more observations at the localisation radius
+//                           than the boundary list holds
+enum : int {
+  MDA_ST_NOT_SPD = 1,
+  MDA_ST_NONFINITE_OBS = 2,
+  MDA_ST_NONFINITE_STATE = 4,
+  MDA_ST_BOUNDARY_OVERFLOW = 8,
+  MDA_ST_SKIP_UPDATE = MDA_ST_NOT_SPD | MDA_ST_NONFINITE_OBS | MDA_ST_BOUNDARY_OVERFLOW,
+};
 
 // ---------------------------------------------------------------------------
 // context
@@ -71,6 +90,9 @@ struct mda_ctx {
     if (bytes == 0) bytes = 256;
     buf& b = ws[name];
     if (b.bytes < bytes) {
+      // a cached graph may hold the old address: drop them all (their keys
+      // hold every address they use, so none would match again anyway)
+      drop_graphs();
       if (b.ptr) MDA_HIP(hipFree(b.ptr));
       b.ptr = nullptr;
       b.bytes = 0;
@@ -83,6 +105,11 @@ struct mda_ctx {
   template <class T>
   T* get(const std::string& name, size_t count) {
     return static_cast<T*>(get(name, count * sizeof(T)));
+  }
+  void drop_graphs() {
+    for (auto& kv : graphs)
+      if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
   }
 };
 
@@ -149,7 +176,8 @@ void launch_obs_stats(hipStream_t s, const double* X, int k, int64_t n, int64_t 
                       const int64_t* row_ptr, const int64_t* col, const double* w,
                       const double* hx_mm, const double* yo, double* ybar, double* yp_om,
                       double* d, double* yp_mm, const uint32_t* order = nullptr,
-                      float* y32 = nullptr, const uint32_t* opos = nullptr);
+                      float* y32 = nullptr, const uint32_t* opos = nullptr,
+                      int* status = nullptr);
 // keys = grid position of each observation's first stencil tap (groups of
 // four levels share a key range; nxy = points per level), vals = 0..p-1
 void launch_tap_keys(hipStream_t s, const int64_t* row_ptr, const int64_t* col, int64_t p,
@@ -166,9 +194,10 @@ void launch_etkf_solve_ws(hipStream_t s, int k, const double* EB, double* T, dou
                           int* status);
 void launch_enkf_solve_ws(hipStream_t s, int k, const double* EB, const double* F, double* T,
                           double* Minv, double* scratch, int* status);
-// X_a = xbar + scale * X' T for all state points (global T)
+// X_a = xbar + scale * X' T for all state points (global T); skipped when
+// the status word holds a MDA_ST_SKIP_UPDATE bit
 void launch_transform_global(hipStream_t s, double* X, int k, int64_t n, double scale,
-                             const double* T);
+                             const double* T, int* status);
 // EnKF diagnostics
 void launch_enkf_gain_minmax_ws(hipStream_t s, const double* X, int k, int64_t n, double scale,
                                 const double* Minv, const double* yp_om, const double* var,
@@ -237,6 +266,12 @@ struct letkf_gram_args {
   const uint32_t* order;
   uint32_t* queue;
   int nblocks;  // launch size (0: one wave per column)
+  // GEOGRAPHIC boundary list of the pass: (column, observation, chord
+  // decision) of candidates within shell_width of the threshold, decided
+  // exactly by launch_letkf_shell_fixup; its counter is zeroed before the Gram
+  int4* shell;
+  uint32_t* shell_count;
+  int64_t shell_cap;
   // Global index of column 0.  The per-column arrays (col_a/b/c, C, b, nloc)
   // start at that column; col0 only enters where the global index itself
   // matters (grid coordinates of a GRID geometry without coordinates).
@@ -247,6 +282,8 @@ struct letkf_gram_args {
 constexpr int LETKF_CREC_INTS = 96;
 void launch_column_setup(hipStream_t s, const letkf_gram_args& a);
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a);
+// exact decisions of the pass's GEOGRAPHIC boundary list (no-op otherwise)
+void launch_letkf_shell_fixup(hipStream_t s, const letkf_gram_args& a, int* status);
 // per-column cost estimate (observations in the candidate buckets) as
 // descending-order sort keys, and column ids
 void launch_column_cost(hipStream_t s, const letkf_gram_args& a, uint32_t* keys,
@@ -303,10 +340,47 @@ bool letkf_eig_supported(int k);
 void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                       int* flag);
 // X_a = xbar + X' T for the columns [col0, col0 + nc) of X (ncol columns in
-// all; nc < 0: every column); T holds the nc columns' transforms
+// all; nc < 0: every column); T holds the nc columns' transforms.  Skipped
+// when the status word holds a MDA_ST_SKIP_UPDATE bit.
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T, int64_t col0 = 0, int64_t nc = -1);
+                            const double* T, int* status, int64_t col0 = 0, int64_t nc = -1);
 void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,
                        int64_t* out_empty);
+
+// runs f with the context's device current and converts its exceptions into
+// MDA_E_* codes (message in ctx->err)
+int guarded_call(mda_ctx* ctx, const std::function<void()>& f);
+
+// ETKF (ETKF.hpp:100-178) on device buffers: Y' [p][k], d [p], Gram
+// partials [ETKF_GG_BLOCKS][k][k+1], [E | b] [k][k+1], T [k][k], Cholesky
+// scratch [2][k][k] and the status word.  etkf_run takes them from the
+// context workspace when `bufs` is null.
+constexpr int ETKF_GG_BLOCKS = 256;
+struct etkf_bufs {
+  double *yp, *dd, *part, *EB, *T, *scr;
+  int* status;
+};
+void etkf_run(mda_ctx* c, double inflation, mda_ensemble* ens, const mda_obs* obs,
+              const mda_hop* hop, const etkf_bufs* bufs);
+
+// dense p x p algebra of the global filters (p <= MDA_DENSE_MAX)
+constexpr int64_t MDA_DENSE_MAX = 4096;
+struct dense_extremes {
+  double lmin, lmax;      // smallest and largest eigenvalue
+  double absmin, absmax;  // smallest and largest |eigenvalue| (singular values)
+};
+void launch_form_enkf_S(hipStream_t s, const double* yp, const double* var, int k, int64_t p,
+                        double* S);
+void dense_tridiagonalise(hipStream_t s, int64_t p, double* A, double* d, double* e, double* u,
+                          double* pv, double* w, double* sc);
+dense_extremes tridiag_extremes(const std::vector<double>& d, const std::vector<double>& e);
+// extreme eigenvalues of the symmetric device matrix A (destroyed)
+dense_extremes dense_sym_extremes(mda_ctx* c, double* A, int64_t p);
+// cond(S) of S = Y'Y'^T/(k-1) + diag(var); NaN beyond MDA_DENSE_MAX
+double dense_spd_condition(mda_ctx* c, const double* yp, const double* var, int k, int64_t p);
+// B <- A^-1 B by Gauss-Jordan with partial pivoting (A destroyed); a zero
+// pivot sets MDA_ST_NOT_SPD in *status
+void dense_gauss_jordan(hipStream_t s, int64_t p, int64_t m, double* A, double* B,
+                        int64_t* piv, double* pval, int* status);
 
 }  // namespace mda

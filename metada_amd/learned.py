@@ -6,18 +6,22 @@ to the prior inflation rho_g of that column's LETKF analysis
 (mda_letkf_params.rho_column, Hunt et al. 2007: A = (k-1)/rho_g I + ...).
 
 It is trained on the plain filter's own cycled twin experiment: a cycled
-Lorenz-96 LETKF with fixed inflation records, before every analysis, the
-features of every column and the inflation that would have made the local
-background spread match the local background error against the truth,
-    rho*_g = sum_j c_gj (xbar_j - x_t,j)^2 / sum_j c_gj s_j^2,
+Lorenz-96 LETKF with fixed inflation rho records, before every analysis, the
+features of every column (the last one is the inflation the previous
+analysis applied) and, as target, the inflation that analysis should have
+applied for the local background spread to match the local background
+error against the truth,
+    target_g = rho sum_j c_gj (xbar_j - x_t,j)^2 / sum_j c_gj s_j^2,
 with c the Gaspari-Cohn weights of the localisation (clipped to [0, 4]
 against outliers; the network's rho >= 1 cannot follow targets below 1 and
-settles at 1 there).  Training (forward,
+settles at 1 there).  Runs at several fixed rho give it both under- and
+over-dispersive backgrounds.  Training (forward,
 backward and Adam) runs in the library's own HIP kernels (mda_mlp_*).  At
 evaluation the learned and the fixed-inflation filters are cycled side by
 side from the same state; the learned one is kept only if its analysis RMSE
-against the truth (Metrics.hpp:265-273) is not worse, otherwise the result
-falls back to fixed inflation.
+against the truth (Metrics.hpp:265-273) is not worse on a validation
+segment, otherwise the result falls back to fixed inflation, and the choice
+is then scored on later cycles it has not seen.
 
 Lorenz-96 (tests/python/lorenz95.py:3-37) is stepped by the library's GPU
 RK4 kernel (mda_l96_forecast); the N grid points sit on the equator, so the
@@ -95,10 +99,11 @@ class Inflator:
     def train(self, feat, target, steps=3000, batch=32768, lr=3e-3, seed=0):
         """Adam on mini-batches of (standardised features, target rho)."""
         torch = _torch()
-        nf = int((~torch.isfinite(feat)).sum())
+        bad_f = (~torch.isfinite(feat)).sum(0)
         nt = int((~torch.isfinite(target)).sum())
-        if nf or nt:
-            raise ValueError(f"training data not finite: {nf} feature and {nt} target values")
+        if int(bad_f.sum()) or nt:
+            raise ValueError(f"training data not finite: {bad_f.tolist()} values per feature, "
+                             f"{nt} targets")
         self.fit_normalisation(feat)
         x_all = self.normalise(feat)
         t_all = target.float().contiguous()
@@ -148,6 +153,7 @@ class CycleLog:
     rmse: list = field(default_factory=list)
     spread: list = field(default_factory=list)
     rho_mean: list = field(default_factory=list)
+    diverged_at: int | None = None
 
 
 class L96Twin:
@@ -195,11 +201,14 @@ class L96Twin:
         return Observation(y.cpu().numpy(), self.var, self.obs_lat, self.obs_lon,
                            coord=L.COORD_GEOGRAPHIC)
 
-    def config(self):
-        return Config({"inflation": self.s.rho_fixed, "localization_radius": self.s.radius_km})
+    def config(self, rho=None):
+        return Config({"inflation": self.s.rho_fixed if rho is None else rho,
+                       "localization_radius": self.s.radius_km})
 
-    def target_rho(self, X):
-        """rho*_g from the background and the truth (see module docstring)."""
+    def target_rho(self, X, rho_prev):
+        """The inflation the last analysis should have applied to column g:
+        rho_prev_g (what it applied) times the background's error-to-spread
+        ratio rho*_g against the truth (see module docstring)."""
         torch = _torch()
         mean = X.mean(0)
         err = (mean - self.truth[0]) ** 2
@@ -209,11 +218,13 @@ class L96Twin:
         pad = lambda v: torch.cat([v[-half:], v, v[:half]]).view(1, 1, -1)
         e = torch.nn.functional.conv1d(pad(err), ke).view(-1)
         s = torch.nn.functional.conv1d(pad(var), ke).view(-1)
-        # no lower clip: where the background is over-dispersive the target is
-        # below 1 and pulls the network's output (>= 1) down to 1
-        return (e / s.clamp_min(1e-12)).clamp(0.0, 4.0)
+        # no lower clip at 1: where the background is over-dispersive the
+        # target is below 1 and pulls the network's output (>= 1) down to 1
+        return (rho_prev * e / s.clamp_min(1e-12)).clamp(0.0, 4.0)
 
-    def features(self, y, cfg):
+    def features(self, y, cfg, rho_prev):
+        """The eight local statistics of every column; rho_prev (device,
+        FP64, [N]) is the inflation the previous analysis applied."""
         torch = _torch()
         f = LETKF(Ensemble(self.X, self.s.N, 1, 1), self.obs(y), self.H, cfg,
                   geometry=self.geom, ctx=self.ctx)
@@ -223,9 +234,12 @@ class L96Twin:
         gs, gk = self.geom.struct()
         os_, ok = _obs_struct(f.obs)
         hs, hk = self.H.struct()
+        prm = f.params
+        prm.rho_on_device = 1  # rho_prev is a device array
+        rp = rho_prev.double().contiguous()
         self.ctx.check(self.ctx.lib.mda_letkf_inflation_features(
-            self.ctx.h, C.byref(f.params), C.byref(es), C.byref(gs), C.byref(os_), C.byref(hs),
-            None, out.data_ptr(), 1))
+            self.ctx.h, C.byref(prm), C.byref(es), C.byref(gs), C.byref(os_), C.byref(hs),
+            rp.data_ptr(), out.data_ptr(), 1))
         del ek, gk, ok, hk  # staging buffers stay alive until the call returns
         return out
 
@@ -258,75 +272,162 @@ def _gc(z):
 
 
 def cycle(twin: L96Twin, ncycles: int, inflator: Inflator | None = None, collect=False,
-          log_every=0, progress=None):
+          log_every=0, progress=None, rho: float | None = None):
     """Forecast, observe, (features, target), analyse; returns the log and,
-    with collect, the stacked features and targets of every cycle."""
+    with collect, the stacked features and targets of every cycle.  `rho`
+    overrides the setup's fixed inflation.  A filter whose ensemble turns
+    non-finite (catastrophic divergence of the cycled model) stops there:
+    the library reports it (MDA_E_NUMERIC) and the log records the cycle in
+    `diverged_at`, with an infinite RMSE for the remaining cycles."""
     torch = _torch()
-    cfg = twin.config()
+    cfg = twin.config(rho)
     lg = CycleLog()
     feats, targets = [], []
+    # inflation applied by the previous analysis, per column (a feature, and
+    # the scale of the training target): the fixed rho, or the learned one
+    rho_prev = torch.full((twin.s.N,), cfg.Get("inflation").asFloat(), device=twin.device,
+                          dtype=torch.float64)
     for c in range(ncycles):
         twin.forecast(twin.truth)
         twin.forecast(twin.X)
         y = twin.observe()
         rho_col = None
-        if collect or inflator is not None:
-            f = twin.features(y, cfg)
-            if collect:
-                feats.append(f)
-                targets.append(twin.target_rho(twin.X).float())
-            if inflator is not None:
-                rho_col = inflator.predict(f)
-                lg.rho_mean.append(float(rho_col.mean()))
-        twin.analyse(y, cfg, rho_col)
+        try:
+            if collect or inflator is not None:
+                f = twin.features(y, cfg, rho_prev)
+                if collect:
+                    feats.append(f)
+                    targets.append(twin.target_rho(twin.X, rho_prev).float())
+                if inflator is not None:
+                    rho_col = inflator.predict(f)
+                    lg.rho_mean.append(float(rho_col.mean()))
+                    rho_prev = rho_col.double()
+            twin.analyse(y, cfg, rho_col)
+        except L.MdaError as e:
+            if e.code != L.MDA_E_NUMERIC:
+                raise
+            lg.diverged_at = c
+        if lg.diverged_at is None and not math.isfinite(twin.rmse()):
+            lg.diverged_at = c
+        if lg.diverged_at is not None:
+            if collect and feats:  # the diverging cycle's inputs are not training data
+                feats.pop()
+                targets.pop()
+            lg.rmse += [math.inf] * (ncycles - c)
+            lg.spread += [math.inf] * (ncycles - c)
+            if progress:
+                progress(f"filter diverged at cycle {c + 1}")
+            break
         lg.rmse.append(twin.rmse())
         lg.spread.append(float(twin.X.var(0, unbiased=True).mean().sqrt()))
         if progress and log_every and (c + 1) % log_every == 0:
             progress(f"cycle {c + 1}/{ncycles}: rmse {lg.rmse[-1]:.4f} spread {lg.spread[-1]:.4f}")
     if collect:
+        if not feats:
+            return lg, None, None
         return lg, torch.cat(feats), torch.cat(targets)
     return lg
 
 
+def _mean(v):
+    return float(np.mean(v)) if len(v) else math.nan
+
+
 def experiment(setup: L96Setup | None = None, train_cycles=1000, eval_cycles=1000,
-               burn_in=100, train_steps=3000, ctx: Context | None = None, progress=None):
-    """The twin-experiment protocol: plain fixed-rho cycles for training data,
-    MLP training on the GPU, then fixed vs learned cycling from the same
-    state with fixed-rho fallback."""
+               burn_in=100, train_steps=3000, val_cycles=200, train_rhos=None,
+               fixed_rhos=None, ctx: Context | None = None, progress=None):
+    """The twin-experiment protocol.
+
+    Training data: the plain LETKF cycled with fixed inflation for
+    `train_cycles` cycles from one initial state, once per rho in
+    `train_rhos` (default: the setup's rho), the samples of cycles after
+    `burn_in`.  The network is trained on all of them with the library's MLP
+    kernels.  Then, for every fixed rho in `fixed_rhos` (default: the setup's
+    rho), from the state of that rho's plain filter at cycle `train_cycles`:
+      validation  cycles train_cycles + 1 .. + val_cycles, fixed and learned
+                  side by side from the same state: the learned inflation is
+                  deployed only if its analysis RMSE against the truth
+                  (Metrics.hpp:265-273) is not worse there, else fixed rho;
+      evaluation  the remaining eval_cycles - val_cycles cycles, each filter
+                  continuing from its own state; the deployed filter's RMSE
+                  is the reported result (the choice never sees these cycles).
+    Returns one record per fixed rho."""
     torch = _torch()
     s = setup or L96Setup()
     ctx = ctx or Context(0)
+    train_rhos = list(train_rhos or [s.rho_fixed])
+    fixed_rhos = list(fixed_rhos or [s.rho_fixed])
     twin = L96Twin(ctx, s)
-    lg0, feat, target = cycle(twin, train_cycles, collect=True,
-                              log_every=max(1, train_cycles // 5), progress=progress)
-    nb = burn_in * s.N
-    feat, target = feat[nb:], target[nb:]
+    st0 = twin.clone_state()
+    feats, targets, runs, end_state = [], [], {}, {}
+    for r in sorted(set(train_rhos) | set(fixed_rhos)):
+        twin.set_state(st0)
+        collect = r in train_rhos
+        if progress:
+            progress(f"plain filter rho={r}: {train_cycles} cycles" + (" (training data)" if collect else ""))
+        out = cycle(twin, train_cycles, collect=collect, log_every=max(1, train_cycles // 4),
+                    progress=progress, rho=r)
+        lg = out[0] if collect else out
+        runs[r] = lg
+        end_state[r] = None if lg.diverged_at is not None else twin.clone_state()
+        if collect and out[1] is not None:
+            nb = burn_in * s.N
+            if out[1].shape[0] > nb:
+                feats.append(out[1][nb:])
+                targets.append(out[2][nb:])
+    if not feats:
+        raise RuntimeError("no training data: every plain filter diverged during burn-in")
+    feat, target = torch.cat(feats), torch.cat(targets)
     inf = Inflator(ctx, seed=s.seed)
     hist = inf.train(feat, target, steps=train_steps)
     if progress:
         progress(f"trained on {feat.shape[0]} samples: loss {hist[0]:.4f} -> {hist[-1]:.4f}")
-    st = twin.clone_state()
-    lg_fixed = cycle(twin, eval_cycles, log_every=max(1, eval_cycles // 4), progress=progress)
-    twin.set_state(st)
-    lg_learn = cycle(twin, eval_cycles, inflator=inf, log_every=max(1, eval_cycles // 4),
-                     progress=progress)
-    r_fixed = float(np.mean(lg_fixed.rmse))
-    r_learn = float(np.mean(lg_learn.rmse))
-    use_learned = r_learn <= r_fixed
-    return {
-        "setup": s.__dict__ | {"radius_km": s.radius_km},
-        "train_cycles": train_cycles, "eval_cycles": eval_cycles, "burn_in": burn_in,
-        "train_samples": int(feat.shape[0]), "train_loss_first": hist[0],
-        "train_loss_last": hist[-1],
-        "rmse_fixed": r_fixed, "rmse_learned": r_learn,
-        "spread_fixed": float(np.mean(lg_fixed.spread)),
-        "spread_learned": float(np.mean(lg_learn.spread)),
-        "rho_learned_mean": float(np.mean(lg_learn.rho_mean)) if lg_learn.rho_mean else None,
-        "rmse_train_fixed": float(np.mean(lg0.rmse[burn_in:])),
-        "fallback": not use_learned,
-        "rmse_deployed": r_learn if use_learned else r_fixed,
-        "gain": (r_fixed - r_learn) / r_fixed,
-    }
+    results = []
+    for r in fixed_rhos:
+        rec = {"setup": s.__dict__ | {"radius_km": s.radius_km, "rho_fixed": r},
+               "train_rhos": train_rhos, "train_cycles": train_cycles,
+               "eval_cycles": eval_cycles, "val_cycles": val_cycles, "burn_in": burn_in,
+               "train_samples": int(feat.shape[0]), "train_loss_first": hist[0],
+               "train_loss_last": hist[-1],
+               "rmse_train_fixed": _mean(runs[r].rmse[burn_in:]),
+               "train_diverged_at": runs[r].diverged_at}
+        if end_state[r] is None:
+            rec.update({"rmse_fixed": math.inf, "rmse_learned": None, "fallback": None,
+                        "rmse_deployed": None, "gain": None})
+            results.append(rec)
+            continue
+        ev = eval_cycles - val_cycles
+        twin.set_state(end_state[r])
+        fv = cycle(twin, val_cycles, rho=r)
+        st_fixed = twin.clone_state()
+        twin.set_state(end_state[r])
+        lv = cycle(twin, val_cycles, inflator=inf, rho=r)
+        st_learn = twin.clone_state()
+        twin.set_state(st_fixed)
+        fe = cycle(twin, ev, rho=r)
+        if lv.diverged_at is None:
+            twin.set_state(st_learn)
+            le = cycle(twin, ev, inflator=inf, rho=r)
+        else:
+            le = CycleLog(rmse=[math.inf] * ev, diverged_at=0)
+        use_learned = _mean(lv.rmse) <= _mean(fv.rmse)
+        r_fixed, r_learn = _mean(fe.rmse), _mean(le.rmse)
+        deployed = r_learn if use_learned else r_fixed
+        rec.update({
+            "rmse_val_fixed": _mean(fv.rmse), "rmse_val_learned": _mean(lv.rmse),
+            "rmse_fixed": r_fixed, "rmse_learned": r_learn,
+            "spread_fixed": _mean(fe.spread), "spread_learned": _mean(le.spread),
+            "rho_learned_mean": _mean(lv.rho_mean + le.rho_mean),
+            "learned_diverged_at": lv.diverged_at if lv.diverged_at is not None else le.diverged_at,
+            "fallback": not use_learned, "rmse_deployed": deployed,
+            "gain": (r_fixed - deployed) / r_fixed,
+            "gain_learned": (r_fixed - r_learn) / r_fixed})
+        if progress:
+            progress(f"rho={r}: eval rmse fixed {r_fixed:.4f} learned {r_learn:.4f} "
+                     f"(validation {rec['rmse_val_fixed']:.4f} / {rec['rmse_val_learned']:.4f}, "
+                     f"{'learned' if use_learned else 'fixed'} deployed)")
+        results.append(rec)
+    return results
 
 
 if __name__ == "__main__":
@@ -338,12 +439,15 @@ if __name__ == "__main__":
     ap.add_argument("--k", type=int, default=40)
     ap.add_argument("--train-cycles", type=int, default=1000)
     ap.add_argument("--eval-cycles", type=int, default=1000)
-    ap.add_argument("--rho", type=float, default=1.05)
+    ap.add_argument("--val-cycles", type=int, default=200)
+    ap.add_argument("--rho", type=float, nargs="+", default=[1.05])
+    ap.add_argument("--train-rho", type=float, nargs="+", default=None)
     ap.add_argument("--support", type=float, default=16.0)
     ap.add_argument("--steps", type=int, default=3000)
     a = ap.parse_args()
     log = lambda m: print(f"[learned] {m}", file=sys.stderr, flush=True)
-    res = experiment(L96Setup(N=a.N, k=a.k, rho_fixed=a.rho, support_gp=a.support),
+    res = experiment(L96Setup(N=a.N, k=a.k, rho_fixed=a.rho[0], support_gp=a.support),
                      train_cycles=a.train_cycles, eval_cycles=a.eval_cycles,
-                     train_steps=a.steps, progress=log)
+                     val_cycles=a.val_cycles, train_steps=a.steps,
+                     train_rhos=a.train_rho or a.rho, fixed_rhos=a.rho, progress=log)
     print(json.dumps(res), flush=True)

@@ -10,6 +10,8 @@
  *   orc_cholesky_lower, orc_lu_inverse, orc_jacobi_eigh, orc_singular_values
  *       -> numpy/scipy fixtures (tests/golden) + reconstruction invariants
  *   orc_haversine_km      -> quarter meridian = pi*R/2, antipodes, symmetry
+ *   orc_grid_distance, orc_cart_distance
+ *                         -> Pythagorean KATs (3-4-5, 1-2-2, 3-4-12), symmetry
  *   orc_gaspari_cohn      -> GC(0)=1, GC(2)=0, continuity at 1, closed-form
  *                            integral check
  *   orc_idw4 / stencil    -> brute-force literal formula, exact-hit KAT
@@ -19,10 +21,15 @@
  *   orc_etkf_analyse      -> Wa Wa^T = (k-1) Pa; mean update vs Kalman gain
  *   orc_letkf_standard    -> W^2 = (k-1) A^-1, W = W^T, W 1 = sqrt(rho) 1,
  *                            empty-column closed form, infinite-radius limit
- *                            equals a global ETKF (symmetric form)
+ *                            equals a global ETKF (symmetric form); one
+ *                            observation at d = r/2, r and beyond r against
+ *                            the Sherman-Morrison closed form of the mean
+ *                            update (pins the taper scale c = r/2)
  *   orc_letkf_refcompat_* -> at rho=1 and infinite radius equals ETKF compat
  *   orc_metrics           -> MetricsTest.cpp:64-195 known answers
- *   orc_l96_*             -> stationary point x=F, RK4 order check
+ *   orc_l96_*             -> stationary point x=F, a hand-computed 5-variable
+ *                            tendency, energy conservation of the advection
+ *                            term (sum x_i a_i = 0), RK4 order check
  *   orc_mlp_*             -> central finite differences
  */
 #include "oracle.h"

@@ -152,6 +152,10 @@ def grid_distance(x1, y1, x2, y2):
     return lib().orc_grid_distance(x1, y1, x2, y2)
 
 
+def cart_distance(x1, y1, z1, x2, y2, z2):
+    return lib().orc_cart_distance(x1, y1, z1, x2, y2, z2)
+
+
 def gaspari_cohn(z):
     return lib().orc_gaspari_cohn(float(z))
 

@@ -97,3 +97,93 @@ def test_batch_rejects_host_inputs(mda, ctx):
                   mda.Config({"inflation": 1.0, "localization_radius": 3.0}), ctx=ctx)
     with pytest.raises(mda._lib.MdaError, match="device-resident"):
         mda.letkf_batch([f])
+
+
+def _device_analysis(mda, ctx, case, prec, radius):
+    """A LETKF over device copies of a host test case (latlon_case)."""
+    import torch
+    dev = "cuda:0"
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    H = case["H"]
+    keep = {"X": t(case["X"]), "yo": t(case["yo"]), "var": t(case["var"]), "lat": t(case["lat"]),
+            "lon": t(case["lon"]), "rp": t(H.row_ptr), "col": t(H.col), "w": t(H.weight),
+            "clat": t(case["col_lat"]), "clon": t(case["col_lon"])}
+    ens = mda.Ensemble(keep["X"], case["nlon"], case["nlat"], case["nlev"])
+    obs = mda.Observation(keep["yo"], keep["var"], keep["lat"], keep["lon"],
+                          coord=mda._lib.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, keep["clat"], keep["clon"])
+    f = mda.LETKF(ens, obs, mda.ObsOperator(keep["rp"], keep["col"], keep["w"]),
+                  mda.Config({"inflation": 1.05, "localization_radius": radius}),
+                  geometry=geom, ctx=ctx, precision=prec)
+    f._keep = keep
+    return f
+
+
+def test_batch_mixed_k_with_k128_matches_oracle(mda, ctx, orc):
+    """The batching form on clustered observations with k = 32, 64 and 128
+    in one batch (hipGraph, work-queue Gram, autotuned launch): every
+    analysis against the oracle's standard LETKF -- FP64 to 1e-10 (max-norm
+    relative), BF16x3 to 1e-4 relative RMS -- and the same batch replayed."""
+    import torch
+    from cases import latlon_case, rel_err
+    L = mda._lib
+    radius = 1500.0
+    specs = [(32, L.PREC_BF16X3), (64, L.PREC_FP64), (128, L.PREC_FP64), (128, L.PREC_BF16X3)]
+    cases = [latlon_case(nlon=32, nlat=16, nlev=3, k=k, p=5000, seed=200 + i, cluster=True)
+             for i, (k, _) in enumerate(specs)]
+    filters = [_device_analysis(mda, ctx, c, prec, radius) for c, (_, prec) in zip(cases, specs)]
+    rho = filters[0].inflation
+    want = []
+    for c in cases:
+        prob = orc.Problem(c["k"], c["nlon"], c["nlat"], c["nlev"], orc.COORD_GEOGRAPHIC,
+                           c["col_lat"], c["col_lon"], c["lat"], c["lon"], c["yo"], c["var"],
+                           rho, radius)
+        Xo, _, nloc = orc.letkf_standard(prob, c["X"], c["H"].apply_host(c["X"]))
+        want.append((Xo, nloc))
+    assert any((nloc == 0).any() for _, nloc in want)  # clustered: empty columns
+    flags = L.BATCH_GRAPH | L.BATCH_QUEUE | L.BATCH_AUTOTUNE
+    for it in range(2):
+        for f, c in zip(filters, cases):
+            f._keep["X"].copy_(torch.from_numpy(c["X"]))
+        torch.cuda.synchronize()
+        diags, info = mda.letkf_batch(filters, flags)
+        assert info["graph_replayed"] == it
+        for f, (k, prec), (Xo, nloc), d in zip(filters, specs, want, diags):
+            Xg = f._keep["X"].cpu().numpy()
+            assert d["total_local_obs"] == int(nloc.sum())
+            if prec == L.PREC_FP64:
+                assert rel_err(Xg, Xo) < 1e-10, (k, rel_err(Xg, Xo))
+            else:
+                rms = np.sqrt(np.mean((Xg - Xo) ** 2)) / np.sqrt(np.mean(Xo ** 2))
+                assert rms < 1e-4, (k, rms)
+
+
+def test_batch_graph_key_follows_every_input(mda, ctx):
+    """A cached graph is replayed only for the same buffers and parameters:
+    a new variance array (other values, other address) gives a new graph and
+    the single-analysis result."""
+    import torch
+    from cases import latlon_case
+    L = mda._lib
+    case = latlon_case(nlon=24, nlat=12, nlev=2, k=16, p=1500, seed=77, cluster=True)
+    f = _device_analysis(mda, ctx, case, L.PREC_FP64, 1500.0)
+    x0 = torch.from_numpy(case["X"]).to("cuda:0")
+
+    f.Analyse()  # the single analysis's own workspace exists before any graph
+
+    def batch_then_single(filt):
+        filt._keep["X"].copy_(x0)
+        diags, info = mda.letkf_batch([filt], L.BATCH_GRAPH | L.BATCH_QUEUE)
+        got = filt._keep["X"].clone()
+        filt._keep["X"].copy_(x0)
+        filt.Analyse()
+        return info, got, filt._keep["X"].clone()
+    info, got, ref = batch_then_single(f)
+    assert torch.equal(got, ref)
+    info, got, ref = batch_then_single(f)
+    assert info["graph_replayed"] == 1 and torch.equal(got, ref)
+    # new variances: a new buffer with other values
+    f._keep["var"] = f._keep["var"] * 1.7
+    f.obs.variances = f._keep["var"]
+    info, got, ref = batch_then_single(f)
+    assert info["graph_replayed"] == 0 and torch.equal(got, ref)

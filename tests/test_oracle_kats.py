@@ -155,3 +155,82 @@ def test_l96_rk4_is_fourth_order(orc):
     e1 = np.abs(run(0.02) - ref).max()
     e2 = np.abs(run(0.01) - ref).max()
     assert 12 < e1 / e2 < 20
+
+
+def test_l96_advection_conserves_energy(orc):
+    """The advection term a_i = (x_{i+1} - x_{i-2}) x_{i-1} conserves
+    sum x_i^2: sum_i x_i a_i = 0 for any x (each triple product appears once
+    with each sign under the cyclic shift).  A wrong index offset in the
+    tendency breaks the identity."""
+    rng = np.random.default_rng(12)
+    for N in (5, 6, 40, 97):
+        x = rng.standard_normal(N) * 3
+        F = 8.0
+        adv = orc.l96_tendency(x, F) + x - F
+        assert abs(float(np.dot(x, adv))) < 1e-12 * float(np.dot(x, x)) * np.abs(x).max()
+
+
+def test_l96_tendency_hand_computed(orc):
+    # x = (1, 2, 3, 4, 5), F = 8, worked by hand:
+    # i=0: (x1 - x3) x4 - x0 + F = (2-4)*5 - 1 + 8 = -3
+    # i=1: (x2 - x4) x0 - x1 + F = (3-5)*1 - 2 + 8 =  4
+    # i=2: (x3 - x0) x1 - x2 + F = (4-1)*2 - 3 + 8 = 11
+    # i=3: (x4 - x1) x2 - x3 + F = (5-2)*3 - 4 + 8 = 13
+    # i=4: (x0 - x2) x3 - x4 + F = (1-3)*4 - 5 + 8 = -5
+    d = orc.l96_tendency(np.array([1.0, 2.0, 3.0, 4.0, 5.0]), 8.0)
+    np.testing.assert_array_equal(d, [-3.0, 4.0, 11.0, 13.0, -5.0])
+
+
+def test_cart_distance_known_answers(orc):
+    assert orc.cart_distance(0, 0, 0, 1, 2, 2) == 3.0
+    assert orc.cart_distance(1, 1, 1, 4, 5, 13) == 13.0       # (3, 4, 12)
+    assert orc.cart_distance(-2, 5, 7, -2, 5, 7) == 0.0
+    assert orc.cart_distance(1, 2, 3, 4, 6, 3) == orc.cart_distance(4, 6, 3, 1, 2, 3) == 5.0
+
+
+def _one_column_letkf(orc, dist, radius, k=6, seed=0, sigma=0.7, rho=1.05, nlev=3):
+    """A 1 x 1 GRID column at (0, 0) with one observation at (dist, 0)."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((k, nlev))
+    Yb = rng.standard_normal((k, 1))
+    yo = np.array([0.8])
+    prob = orc.Problem(k, 1, 1, nlev, orc.COORD_GRID, np.zeros(1), np.zeros(1),
+                       np.array([dist]), np.zeros(1), yo, np.array([sigma * sigma]), rho, radius)
+    Xa, _, nloc = orc.letkf_standard(prob, X, Yb)
+    return X, Yb, yo, Xa, int(nloc[0])
+
+
+def _single_obs_mean_increment(X, Yb, yo, w, rho):
+    """Closed form of the standard LETKF mean update with one observation of
+    weight w (GC taper / variance): A = (k-1)/rho I + w y' y'^T, so by
+    Sherman-Morrison A^-1 y' = y' / ((k-1)/rho + w |y'|^2) and
+    xbar_a - xbar = X'^T A^-1 y' w d."""
+    k = X.shape[0]
+    yp = Yb[:, 0] - Yb[:, 0].mean()
+    d = yo[0] - Yb[:, 0].mean()
+    Xp = X - X.mean(0)
+    return Xp.T @ yp * (w * d / ((k - 1) / rho + w * (yp @ yp)))
+
+
+def test_letkf_taper_scale_is_half_the_radius(orc):
+    """GC support = radius, i.e. c = r/2: an observation at d = r/2 has
+    weight GC(1) / sigma^2 = (5/24) / sigma^2 (with c = r it would be
+    GC(1/2) / sigma^2 = 0.6823 / sigma^2), one at d = r has weight 0 but is
+    local, one beyond r is not local."""
+    r, sigma, rho = 10.0, 0.7, 1.05
+    X, Yb, yo, Xa, nloc = _one_column_letkf(orc, r / 2, r, sigma=sigma, rho=rho)
+    assert nloc == 1
+    inc = Xa.mean(0) - X.mean(0)
+    want = _single_obs_mean_increment(X, Yb, yo, (5.0 / 24.0) / sigma ** 2, rho)
+    np.testing.assert_allclose(inc, want, rtol=1e-12, atol=1e-14)
+    wrong = _single_obs_mean_increment(X, Yb, yo, orc.gaspari_cohn(0.5) / sigma ** 2, rho)
+    assert np.max(np.abs(inc - wrong)) > 1e-3 * np.max(np.abs(want))
+    # d = r: zero weight, so only the inflation acts; still a local observation
+    X, Yb, yo, Xa, nloc = _one_column_letkf(orc, r, r, sigma=sigma, rho=rho)
+    assert nloc == 1
+    np.testing.assert_allclose(Xa.mean(0), X.mean(0), rtol=0, atol=1e-13)
+    Xp, Xap = X - X.mean(0), Xa - Xa.mean(0)
+    np.testing.assert_allclose(Xap, np.sqrt(rho) * Xp, rtol=1e-12, atol=1e-14)
+    # beyond r: an empty column (LETKF.hpp:167-190)
+    X, Yb, yo, Xa, nloc = _one_column_letkf(orc, np.nextafter(r, 20.0), r)
+    assert nloc == 0
