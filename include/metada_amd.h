@@ -146,6 +146,19 @@ int mda_enkf_analyse(mda_ctx* ctx, double inflation, int32_t inflation_method,
 int mda_etkf_analyse(mda_ctx* ctx, double inflation, mda_ensemble* ens, const mda_obs* obs,
                      const mda_hop* hop);
 
+/* Allocation-free ETKF.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same result as mda_etkf_analyse; the
+ * ensemble, observations and H must be device-resident, and every other
+ * buffer comes from a static 26 MiB device array per GPU (allocated at module
+ * load), so k <= MDA_NOALLOC_MAX_K and p * (k + 1) <= MDA_NOALLOC_MAX_OBS_VALUES
+ * (MDA_E_ARG beyond).  Calls share that array: calls that overlap on one GPU
+ * race. */
+#define MDA_NOALLOC_MAX_K 64
+#define MDA_NOALLOC_MAX_OBS_VALUES 2097152
+int mda_etkf_analyse_noalloc(mda_ctx* ctx, double inflation, mda_ensemble* ens,
+                             const mda_obs* obs, const mda_hop* hop);
+
 /* ---- LETKF (LETKF.hpp:47-243) --------------------------------------------------- */
 enum { MDA_LOC_GASPARI_COHN = 0, MDA_LOC_CUTOFF = 1 };
 enum { MDA_RW_REAL = 0, MDA_RW_IDENTITY = 1 };

@@ -108,6 +108,8 @@ SIGNATURES = {
                                    C.POINTER(Hop), C.c_void_p, _i32, C.POINTER(EnkfDiag)]),
     "mda_etkf_analyse": (C.c_int, [C.c_void_p, _d, C.POINTER(Ensemble), C.POINTER(Obs),
                                    C.POINTER(Hop)]),
+    "mda_etkf_analyse_noalloc": (C.c_int, [C.c_void_p, _d, C.POINTER(Ensemble), C.POINTER(Obs),
+                                           C.POINTER(Hop)]),
     "mda_letkf_params_standard": (None, [C.POINTER(LetkfParams), _d, _d]),
     "mda_letkf_params_refcompat": (None, [C.POINTER(LetkfParams), _d, _d]),
     "mda_letkf_analyse": (C.c_int, [C.c_void_p, C.POINTER(LetkfParams), C.POINTER(Ensemble),
