@@ -2,20 +2,33 @@
 // metada_amd.h the shape of metada's analysis algorithms, so a backend can
 // switch its analysis step to the MI355X library without touching callers.
 //
-//   metada_amd::LETKF<Tag>   mirrors  LETKF<BackendTag>  (LETKF.hpp:47-125)
-//   metada_amd::ETKF<Tag>    mirrors  ETKF<BackendTag>   (ETKF.hpp:84-178)
-//   metada_amd::EnKF<Tag>    mirrors  EnKF<BackendTag>   (EnKF.hpp:84-256)
+//   metada_amd::LETKF<Tag>   mirrors  LETKF<BackendTag>  (LETKF.hpp:47-142)
+//   metada_amd::ETKF<Tag>    mirrors  ETKF<BackendTag>   (ETKF.hpp:84-200)
+//   metada_amd::EnKF<Tag>    mirrors  EnKF<BackendTag>   (EnKF.hpp:84-301)
 //
-// Same constructor arguments (non-owning references, the analysis config or
-// for EnKF the root config), same Analyse() that updates the ensemble members
-// in place, std::runtime_error on failure (the reference's error behaviour).
-// The adapter is written against the reference's adapter API:
-//   ensemble.Size(), ensemble.GetMember(j).getData<std::vector<double>>(),
-//   ensemble.GetMember(0).geometry() iterable over grid Locations,
-//   obs.getObservationValues(), obs.getCovariance(), range-for over obs
-//   points with .location, obs_op.apply(member, obs) -> std::vector<double>,
-//   config.Get(key).asFloat(), Location::getGeographicCoords() /
-//   getGridCoords() / getCartesianCoords() and Location::system.
+// Same constructor arguments (non-owning references; the analysis config,
+// or for EnKF the root config whose "analysis" subsection it reads), same
+// Analyse() that updates the ensemble members in place and refreshes the
+// ensemble statistics as the reference does (LETKF: RecomputeMean,
+// LETKF.hpp:116; EnKF: RecomputeMean and RecomputePerturbations,
+// EnKF.hpp:237-238), same saveEnsemble() (EnKF.hpp:261-282, LETKF.hpp:125-142)
+// and, for EnKF, getAnalysisResults() with the reference's AnalysisResults
+// fields (EnKF.hpp:84-96).  Errors surface as std::runtime_error, the
+// reference's error behaviour.
+//
+// The adapter is written against the reference's adapter API only:
+//   Ensemble:    Size(), GetMember(i), RecomputeMean(), RecomputePerturbations(),
+//                Mean()                                       (Ensemble.hpp)
+//   State:       getDataPtr<double>(), size(), geometry(), saveToFile(name)
+//                                                             (State.hpp:229-275,383)
+//   Geometry:    range-for over Location                      (Geometry.hpp)
+//   Location:    getCoordinateSystem(), getGridCoords(), getGeographicCoords(),
+//                getCartesianCoords()                         (Location.hpp:100-180)
+//   Observation: getObservationValues(), getCovariance(), size(), range-for
+//                over points with .location                   (Observation.hpp:261-482)
+//   ObsOperator: apply(member, obs) -> std::vector<double>    (ObsOperator.hpp:255-259)
+//   Config:      Get(key) -> ConfigValue with asFloat() / asString(),
+//                GetSubsection(key) -> Config                 (Config.hpp:203-313)
 // H(x_j) comes from the backend's own ObsOperator (MDA_HOP_HX), so every
 // observation operator the reference supports works unchanged.
 #ifndef METADA_AMD_HPP
@@ -60,33 +73,53 @@ class Context {
   mda_ctx* ctx_ = nullptr;
 };
 
+// EnKF<BackendTag>::AnalysisResults (EnKF.hpp:84-96)
+struct AnalysisResults {
+  double innovation_norm;
+  double analysis_increment_norm;
+  double background_spread;
+  double analysis_spread;
+  double max_kalman_gain;
+  double min_kalman_gain;
+  double condition_number;
+  int ensemble_size;
+  int observation_count;
+  std::string inflation_method;
+  double inflation_factor;
+};
+
 namespace detail {
 
-// Observation locations and the coordinate system of a Location-like value.
+// The coordinate system of a Location as the C-ABI's MDA_COORD_*.  Compared
+// by enumerator name, so the reference's enum class needs no include here.
 template <class Loc>
 inline int32_t coord_of(const Loc& l) {
-  using CS = decltype(l.system);
-  if (l.system == CS::GEOGRAPHIC) return MDA_COORD_GEOGRAPHIC;
-  if (l.system == CS::CARTESIAN) return MDA_COORD_CARTESIAN;
+  using CS = decltype(l.getCoordinateSystem());
+  const CS cs = l.getCoordinateSystem();
+  if (cs == CS::GEOGRAPHIC) return MDA_COORD_GEOGRAPHIC;
+  if (cs == CS::CARTESIAN) return MDA_COORD_CARTESIAN;
   return MDA_COORD_GRID;
 }
 
 template <class Loc>
 inline void coords_of(const Loc& l, double& a, double& b, double& c) {
-  using CS = decltype(l.system);
-  if (l.system == CS::GEOGRAPHIC) {
-    auto t = l.getGeographicCoords();  // (lat, lon, level)
+  using CS = decltype(l.getCoordinateSystem());
+  const CS cs = l.getCoordinateSystem();
+  if (cs == CS::GEOGRAPHIC) {
+    const auto t = l.getGeographicCoords();  // (lat, lon, level)
     a = std::get<0>(t); b = std::get<1>(t); c = std::get<2>(t);
-  } else if (l.system == CS::CARTESIAN) {
-    auto t = l.getCartesianCoords();
+  } else if (cs == CS::CARTESIAN) {
+    const auto t = l.getCartesianCoords();
     a = std::get<0>(t); b = std::get<1>(t); c = std::get<2>(t);
   } else {
-    auto t = l.getGridCoords();  // (i, j, k)
+    const auto t = l.getGridCoords();  // (i, j, k)
     a = std::get<0>(t); b = std::get<1>(t); c = std::get<2>(t);
   }
 }
 
-// Host copies of everything the C-ABI needs for one analysis.
+// Host views of everything the C-ABI needs for one analysis.  The members
+// stay where the State objects keep them (SimpleState.hpp:265): the library
+// copies them to the device and writes the analysis back in place.
 struct Staged {
   int k = 0;
   int64_t n = 0, p = 0;
@@ -100,13 +133,19 @@ Staged stage(Ens& ens, Obs& obs, const ObsOp& op) {
   Staged s;
   s.k = static_cast<int>(ens.Size());
   for (int j = 0; j < s.k; ++j) {
-    auto& v = ens.GetMember(j).template getData<std::vector<double>>();
-    s.members.push_back(v.data());
-    s.n = static_cast<int64_t>(v.size());
+    auto& member = ens.GetMember(j);
+    double* data = member.template getDataPtr<double>();  // State.hpp:229-232
+    const int64_t size = static_cast<int64_t>(member.size());
+    if (!data) throw std::runtime_error("ensemble member has no data");
+    if (j > 0 && size != s.n) throw std::runtime_error("ensemble members differ in size");
+    s.members.push_back(data);
+    s.n = size;
   }
   s.yo = obs.getObservationValues();
   s.var = obs.getCovariance();  // diagonal R: error^2 (+inf marks invalid)
   s.p = static_cast<int64_t>(s.yo.size());
+  if (static_cast<int64_t>(s.var.size()) != s.p)
+    throw std::runtime_error("observation values and covariance differ in size");
   bool first = true;
   for (const auto& pt : obs) {
     double a, b, c;
@@ -150,14 +189,23 @@ inline mda_hop hx_view(Staged& s) {
   return h;
 }
 
+// saveEnsemble of the reference (EnKF.hpp:261-282, LETKF.hpp:125-142):
+// <base>_mean.<format> and <base>_member_<i>.<format> through State::saveToFile
+template <class Ens>
+void save_ensemble(Ens& ens, const std::string& base, const std::string& format) {
+  ens.Mean().saveToFile(base + "_mean." + format);
+  for (size_t i = 0; i < ens.Size(); ++i)
+    ens.GetMember(i).saveToFile(base + "_member_" + std::to_string(i) + "." + format);
+}
+
 }  // namespace detail
 
-// LETKF<BackendTag> (LETKF.hpp:47-125).  `form` selects Hunt et al. (2007)
-// with Gaspari-Cohn R-localisation (MDA default) or the reference's own
-// semantics (refcompat preset: R = I, hard cutoff, posterior inflation,
-// Cholesky square root, element-wise combination).  Grid columns come from
-// the first member's geometry (LETKF.hpp:80-102) as nx * ny columns of nlev
-// levels in the reference's linear order.
+// LETKF<BackendTag> (LETKF.hpp:47-142).  `form` selects Hunt et al. (2007)
+// with Gaspari-Cohn R-localisation (Standard) or the reference's own
+// semantics (RefCompat: R = I, hard cutoff, posterior inflation, Cholesky
+// square root, element-wise combination, snapshot H).  Grid columns come
+// from the first member's geometry (LETKF.hpp:80-102) as nx * ny columns of
+// nlev levels in the reference's linear order.
 template <class BackendTag>
 class LETKF {
  public:
@@ -166,13 +214,21 @@ class LETKF {
   template <class Ens, class Obs, class ObsOp, class Cfg>
   LETKF(Ens& ensemble, Obs& obs, const ObsOp& obs_op, const Cfg& config,
         Form form = Form::Standard, Context& ctx = Context::thread_default())
-      : run_([&ensemble, &obs, &obs_op, this] { this->analyse(ensemble, obs, obs_op); }),
-        ctx_(ctx),
+      : ctx_(ctx),
         inflation_(config.Get("inflation").asFloat()),
         radius_(config.Get("localization_radius").asFloat()),
-        form_(form) {}
+        output_base_file_(config.Get("output_base_file").asString()),
+        format_(config.Get("format").asString()),
+        form_(form) {
+    run_ = [&ensemble, &obs, &obs_op, this] {
+      this->analyse(ensemble, obs, obs_op);
+      ensemble.RecomputeMean();  // LETKF.hpp:116
+    };
+    save_ = [&ensemble, this] { detail::save_ensemble(ensemble, output_base_file_, format_); };
+  }
 
   void Analyse() { run_(); }
+  void saveEnsemble() const { save_(); }
   const mda_letkf_diag& diagnostics() const { return diag_; }
   // per-column inflation (learned inflation), nx*ny values; empty = fixed
   void setColumnInflation(std::vector<double> rho) { rho_col_ = std::move(rho); }
@@ -217,26 +273,35 @@ class LETKF {
           "LETKF::Analyse");
   }
 
-  std::function<void()> run_;
+  std::function<void()> run_, save_;
   Context& ctx_;
   double inflation_, radius_;
+  std::string output_base_file_, format_;
   Form form_;
   std::vector<double> rho_col_;
   mda_letkf_diag diag_{};
 };
 
-// ETKF<BackendTag> (ETKF.hpp:84-178): reference semantics (X' *= rho, Cholesky
-// square root, dense real R).
+// ETKF<BackendTag> (ETKF.hpp:84-200): reference semantics (X' *= rho,
+// Cholesky square root, dense real R); the analysis config as LETKF's.
 template <class BackendTag>
 class ETKF {
  public:
   template <class Ens, class Obs, class ObsOp, class Cfg>
   ETKF(Ens& ensemble, Obs& obs, const ObsOp& obs_op, const Cfg& config,
        Context& ctx = Context::thread_default())
-      : run_([&ensemble, &obs, &obs_op, this] { this->analyse(ensemble, obs, obs_op); }),
-        ctx_(ctx),
-        inflation_(config.Get("inflation").asFloat()) {}
+      : ctx_(ctx),
+        inflation_(config.Get("inflation").asFloat()),
+        output_base_file_(config.Get("output_base_file").asString()),
+        format_(config.Get("format").asString()) {
+    run_ = [&ensemble, &obs, &obs_op, this] {
+      this->analyse(ensemble, obs, obs_op);
+      ensemble.RecomputeMean();
+    };
+    save_ = [&ensemble, this] { detail::save_ensemble(ensemble, output_base_file_, format_); };
+  }
   void Analyse() { run_(); }
+  void saveEnsemble() const { save_(); }
 
  private:
   template <class Ens, class Obs, class ObsOp>
@@ -247,26 +312,61 @@ class ETKF {
     mda_hop h = detail::hx_view(s);
     check(ctx_.get(), mda_etkf_analyse(ctx_.get(), inflation_, &e, &o, &h), "ETKF::Analyse");
   }
-  std::function<void()> run_;
+  std::function<void()> run_, save_;
   Context& ctx_;
   double inflation_;
+  std::string output_base_file_, format_;
 };
 
-// EnKF<BackendTag> (EnKF.hpp:84-256): takes the root config (EnKF.hpp:105-109)
-// and the N(0,1) draws Z [p][k] that the reference takes from
-// std::random_device (EnKF.hpp:346-360), here injected for reproducibility.
+// EnKF<BackendTag> (EnKF.hpp:84-301): takes the root config and reads its
+// "analysis" subsection (EnKF.hpp:105-128), and the N(0,1) draws Z [p][k]
+// that the reference takes from std::random_device (EnKF.hpp:346-360),
+// injected here for reproducibility.
 template <class BackendTag>
 class EnKF {
  public:
   template <class Ens, class Obs, class ObsOp, class Cfg>
   EnKF(Ens& ensemble, Obs& obs, const ObsOp& obs_op, const Cfg& root_config,
        std::vector<double> Z, Context& ctx = Context::thread_default())
-      : run_([&ensemble, &obs, &obs_op, this] { this->analyse(ensemble, obs, obs_op); }),
-        ctx_(ctx),
-        inflation_(root_config.Get("analysis").Get("inflation").asFloat()),
-        Z_(std::move(Z)) {}
+      : ctx_(ctx), Z_(std::move(Z)) {
+    const auto analysis_config = root_config.GetSubsection("analysis");
+    inflation_factor_ = analysis_config.Get("inflation").asFloat();
+    output_base_file_ = analysis_config.Get("output_base_file").asString();
+    format_ = analysis_config.Get("format").asString();
+    // additive and relaxation fall back to multiplicative, as do unknown
+    // names (EnKF.hpp:116-128, 316-331)
+    const std::string m = analysis_config.Get("inflation_method").asString();
+    if (m == "additive") {
+      method_ = MDA_INFL_ADDITIVE; method_name_ = m;
+    } else if (m == "relaxation") {
+      method_ = MDA_INFL_RELAXATION; method_name_ = m;
+    } else {
+      method_ = MDA_INFL_MULTIPLICATIVE; method_name_ = "multiplicative";
+    }
+    run_ = [&ensemble, &obs, &obs_op, this] {
+      this->analyse(ensemble, obs, obs_op);
+      ensemble.RecomputeMean();  // EnKF.hpp:237-238
+      ensemble.RecomputePerturbations();
+    };
+    save_ = [&ensemble, this] { detail::save_ensemble(ensemble, output_base_file_, format_); };
+  }
   void Analyse() { run_(); }
-  const mda_enkf_diag& getAnalysisResults() const { return diag_; }
+  void saveEnsemble() const { save_(); }
+  AnalysisResults getAnalysisResults() const {
+    AnalysisResults r;
+    r.innovation_norm = diag_.innovation_norm;
+    r.analysis_increment_norm = diag_.analysis_increment_norm;
+    r.background_spread = diag_.background_spread;
+    r.analysis_spread = diag_.analysis_spread;
+    r.max_kalman_gain = diag_.max_kalman_gain;
+    r.min_kalman_gain = diag_.min_kalman_gain;
+    r.condition_number = diag_.condition_number;
+    r.ensemble_size = diag_.ensemble_size;
+    r.observation_count = static_cast<int>(diag_.observation_count);
+    r.inflation_method = method_name_;
+    r.inflation_factor = inflation_factor_;
+    return r;
+  }
 
  private:
   template <class Ens, class Obs, class ObsOp>
@@ -278,13 +378,15 @@ class EnKF {
     mda_obs o = detail::obs_view(s);
     mda_hop h = detail::hx_view(s);
     check(ctx_.get(),
-          mda_enkf_analyse(ctx_.get(), inflation_, MDA_INFL_MULTIPLICATIVE, &e, &o, &h, Z_.data(),
-                           0, &diag_),
+          mda_enkf_analyse(ctx_.get(), inflation_factor_, method_, &e, &o, &h, Z_.data(), 0,
+                           &diag_),
           "EnKF::Analyse");
   }
-  std::function<void()> run_;
+  std::function<void()> run_, save_;
   Context& ctx_;
-  double inflation_;
+  double inflation_factor_ = 1.0;
+  std::string output_base_file_, format_, method_name_;
+  int32_t method_ = MDA_INFL_MULTIPLICATIVE;
   std::vector<double> Z_;
   mda_enkf_diag diag_{};
 };

@@ -6,15 +6,14 @@ to the prior inflation rho_g of that column's LETKF analysis
 (mda_letkf_params.rho_column, Hunt et al. 2007: A = (k-1)/rho_g I + ...).
 
 It is trained on the plain filter's own cycled twin experiment: a cycled
-Lorenz-96 LETKF with fixed inflation rho records, before every analysis, the
-features of every column (the last one is the inflation the previous
-analysis applied) and, as target, the inflation that analysis should have
-applied for the local background spread to match the local background
-error against the truth,
-    target_g = rho sum_j c_gj (xbar_j - x_t,j)^2 / sum_j c_gj s_j^2,
-with c the Gaspari-Cohn weights of the localisation (clipped to [0, 4]
+Lorenz-96 LETKF with fixed inflation records, before every analysis, the
+features of every column and, as target, the inflation of the current
+background that would make its local spread match its local error against
+the truth,
+    rho*_g = sum_j c_gj (xbar_j - x_t,j)^2 / sum_j c_gj s_j^2,
+with c the Gaspari-Cohn weights of the localisation, clipped to [0, 4]
 against outliers; the network's rho >= 1 cannot follow targets below 1 and
-settles at 1 there).  Runs at several fixed rho give it both under- and
+settles at 1 there.  Runs at several fixed rho give it both under- and
 over-dispersive backgrounds.  Training (forward,
 backward and Adam) runs in the library's own HIP kernels (mda_mlp_*).  At
 evaluation the learned and the fixed-inflation filters are cycled side by
@@ -75,8 +74,13 @@ class Inflator:
         return ((feat - self.mu) / self.sd).contiguous()
 
     def fit_normalisation(self, feat):
+        torch = _torch()
         self.mu = feat.mean(0)
-        self.sd = feat.std(0).clamp_min(1e-6)
+        sd = feat.std(0)
+        # a feature constant over the training data (e.g. the previous
+        # inflation when one fixed rho made all of it) keeps unit scale, so
+        # other values at inference stay of order one
+        self.sd = torch.where(sd > 1e-6, sd, torch.ones_like(sd))
 
     def predict(self, feat):
         torch = _torch()
@@ -205,10 +209,10 @@ class L96Twin:
         return Config({"inflation": self.s.rho_fixed if rho is None else rho,
                        "localization_radius": self.s.radius_km})
 
-    def target_rho(self, X, rho_prev):
-        """The inflation the last analysis should have applied to column g:
-        rho_prev_g (what it applied) times the background's error-to-spread
-        ratio rho*_g against the truth (see module docstring)."""
+    def target_rho(self, X):
+        """rho*_g: the background's error-to-spread ratio against the truth
+        around column g (see module docstring), the inflation of the current
+        background that would make its spread match its error."""
         torch = _torch()
         mean = X.mean(0)
         err = (mean - self.truth[0]) ** 2
@@ -220,11 +224,10 @@ class L96Twin:
         s = torch.nn.functional.conv1d(pad(var), ke).view(-1)
         # no lower clip at 1: where the background is over-dispersive the
         # target is below 1 and pulls the network's output (>= 1) down to 1
-        return (rho_prev * e / s.clamp_min(1e-12)).clamp(0.0, 4.0)
+        return (e / s.clamp_min(1e-12)).clamp(0.0, 4.0)
 
-    def features(self, y, cfg, rho_prev):
-        """The eight local statistics of every column; rho_prev (device,
-        FP64, [N]) is the inflation the previous analysis applied."""
+    def features(self, y, cfg):
+        """The eight local statistics of every column (mda_letkf_inflation_features)."""
         torch = _torch()
         f = LETKF(Ensemble(self.X, self.s.N, 1, 1), self.obs(y), self.H, cfg,
                   geometry=self.geom, ctx=self.ctx)
@@ -234,12 +237,9 @@ class L96Twin:
         gs, gk = self.geom.struct()
         os_, ok = _obs_struct(f.obs)
         hs, hk = self.H.struct()
-        prm = f.params
-        prm.rho_on_device = 1  # rho_prev is a device array
-        rp = rho_prev.double().contiguous()
         self.ctx.check(self.ctx.lib.mda_letkf_inflation_features(
-            self.ctx.h, C.byref(prm), C.byref(es), C.byref(gs), C.byref(os_), C.byref(hs),
-            rp.data_ptr(), out.data_ptr(), 1))
+            self.ctx.h, C.byref(f.params), C.byref(es), C.byref(gs), C.byref(os_), C.byref(hs),
+            None, out.data_ptr(), 1))
         del ek, gk, ok, hk  # staging buffers stay alive until the call returns
         return out
 
@@ -283,10 +283,6 @@ def cycle(twin: L96Twin, ncycles: int, inflator: Inflator | None = None, collect
     cfg = twin.config(rho)
     lg = CycleLog()
     feats, targets = [], []
-    # inflation applied by the previous analysis, per column (a feature, and
-    # the scale of the training target): the fixed rho, or the learned one
-    rho_prev = torch.full((twin.s.N,), cfg.Get("inflation").asFloat(), device=twin.device,
-                          dtype=torch.float64)
     for c in range(ncycles):
         twin.forecast(twin.truth)
         twin.forecast(twin.X)
@@ -294,14 +290,13 @@ def cycle(twin: L96Twin, ncycles: int, inflator: Inflator | None = None, collect
         rho_col = None
         try:
             if collect or inflator is not None:
-                f = twin.features(y, cfg, rho_prev)
+                f = twin.features(y, cfg)
                 if collect:
                     feats.append(f)
-                    targets.append(twin.target_rho(twin.X, rho_prev).float())
+                    targets.append(twin.target_rho(twin.X).float())
                 if inflator is not None:
                     rho_col = inflator.predict(f)
                     lg.rho_mean.append(float(rho_col.mean()))
-                    rho_prev = rho_col.double()
             twin.analyse(y, cfg, rho_col)
         except L.MdaError as e:
             if e.code != L.MDA_E_NUMERIC:

@@ -110,17 +110,36 @@ def test_trilinear_reproduces_affine_fields():
     assert np.allclose(np.diff(H.row_ptr), 8)
 
 
-def test_cpp_adapter_compiles_against_reference_shaped_types():
-    """include/metada_amd.hpp instantiated with stand-ins that have the
-    reference adapter API's member names (tests/cpp/adapter_mock.cpp)."""
+def build_adapter_mock(out_dir):
+    """Compiles and links tests/cpp/adapter_mock.cpp (the C++ adapter over
+    stand-ins with the reference API's signatures) against the library and
+    the oracle (its IDW4 observation operator).  Returns the executable."""
     import shutil
     import subprocess
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
-                        "-I", os.path.join(root, "include"),
-                        os.path.join(root, "tests", "cpp", "adapter_mock.cpp")],
+    from oracle import oracle as orc
+    orc.build()
+    libdir, orcdir = os.path.join(root, "metada_amd"), os.path.join(root, "oracle")
+    exe = os.path.join(str(out_dir), "adapter_mock")
+    r = subprocess.run([gxx, "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(root, "include"), "-I", orcdir,
+                        os.path.join(root, "tests", "cpp", "adapter_mock.cpp"), "-o", exe,
+                        "-L", libdir, "-lmetada_amd", "-L", orcdir, "-loracle",
+                        f"-Wl,-rpath,{libdir}", f"-Wl,-rpath,{orcdir}"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_cpp_adapter_builds_against_reference_shaped_types(tmp_path):
+    """include/metada_amd.hpp instantiated and linked with stand-ins that
+    reproduce the reference adapter API's signatures (getDataPtr, the
+    void*-backed getData, ConfigValue-returning Get, GetSubsection, the
+    private Location coordinate system)."""
+    if not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
+            __file__))), "metada_amd", "libmetada_amd.so")):
+        pytest.skip("library not built")
+    assert os.access(build_adapter_mock(tmp_path), os.X_OK)
