@@ -49,7 +49,11 @@ typedef struct mda_ctx mda_ctx;
  * (ApplicationContext.hpp:84-89 initialises config/logging only). */
 int mda_ctx_create(int device, mda_ctx** out);
 int mda_ctx_destroy(mda_ctx* ctx);
+/* Message of the context's last failed call; with ctx == NULL, of this host
+ * thread's last failed context-free call (the file functions accept a NULL
+ * context). */
 const char* mda_last_error(const mda_ctx* ctx);
+const char* mda_last_io_error(void);
 int mda_ctx_synchronize(mda_ctx* ctx);
 /* hipStream_t of the context, as void* (callers that interleave their own
  * kernels order them against this stream). */
@@ -265,6 +269,65 @@ int mda_letkf_last_local_counts(mda_ctx* ctx, int32_t* counts);
  * bits say why (1 QL non-convergence, 2 eigen-residual, 4 orthogonality of a
  * close eigenpair, 8 eigenvalue below (k-1)/rho). */
 int mda_letkf_last_column_flags(mda_ctx* ctx, int32_t* flags);
+
+/* ---- observation operator and files around the analysis ------------------------ */
+/* IdentityObsOperator (IdentityObsOperator.hpp:154-281,483-676) built on the
+ * device as a CSR stencil (MDA_HOP_CSR): four taps per observation, weights
+ * w / sum(w) with w = 1/dist (1e12 at dist 0), the neighbours
+ * find4NearestGridPoints selects (:594-638) on an nx x ny x nlev grid.
+ *   obs_coord GRID        the observation's (x, y, z) grid coordinates;
+ *   obs_coord GEOGRAPHIC  mapped to a grid point first (convertGeographicToGrid,
+ *                         :483-587): with a GEOGRAPHIC geometry, the nearest
+ *                         column in (lat, lon) degrees (first on ties, y-major
+ *                         order) and the nearest of the nlev levels vcoord
+ *                         (NULL: level 0); with any other geometry (a
+ *                         SimpleGeometry), every observation maps to (0, 0, 0),
+ *                         as the reference's fallback search leaves it.
+ * An invalid observation (variance +inf) gets an empty row, so H x = 0 there
+ * (:164-167).  row_ptr [p+1], col and weight [4 p] (at most), on the device
+ * when out_on_device. */
+int mda_hop_idw4_build(mda_ctx* ctx, const mda_obs* obs, int32_t obs_coord,
+                       const mda_geometry* geom, int64_t nx, int64_t ny, int64_t nlev,
+                       const double* vcoord, int64_t* row_ptr, int64_t* col, double* weight,
+                       int32_t out_on_device);
+
+/* The file functions below run on the host; ctx may be NULL (the error
+ * message is then mda_last_error(NULL)).
+ *
+ * GridObservation::loadFromFile (GridObservation.hpp:401-476): the text
+ * format "Time Z Y X ZNU XLONG_U XLAT_U Value" after its header and dashed
+ * separator; values equal to missing_value are dropped; variance = error^2
+ * (the config layer narrows error and missing_value through float,
+ * GridObservation.hpp:85-86); locations (X, Y, Z) for MDA_COORD_GRID or
+ * (XLAT_U, XLONG_U, ZNU) for MDA_COORD_GEOGRAPHIC.  The set owns host arrays,
+ * viewed as an mda_obs until mda_obs_set_free. */
+typedef struct mda_obs_set mda_obs_set;
+int mda_obs_read_grid_file(mda_ctx* ctx, const char* path, double error, double missing_value,
+                           int32_t coord, mda_obs_set** out);
+int mda_obs_set_view(const mda_obs_set* set, mda_obs* view, int32_t* coord);
+void mda_obs_set_free(mda_obs_set* set);
+
+/* SimpleState::readFromFile (SimpleState.hpp:248-263): every number of the
+ * file in order; *count = how many (out may be NULL to size the buffer). */
+int mda_state_read_text(mda_ctx* ctx, const char* path, double* out, int64_t capacity,
+                        int64_t* count);
+/* SimpleState::saveToFile (SimpleState.hpp:162-188): nrows rows of nx values,
+ * "%12.6f" separated by one space -- six decimals, so lossy (about 5e-7). */
+int mda_state_write_text(mda_ctx* ctx, const char* path, const double* v, int64_t nx,
+                         int64_t nrows);
+/* FP64-exact sidecar of a state (no counterpart in the reference, whose text
+ * output is lossy): a 64-byte header ("MDASTATE", version 1, nx, ny, nlev as
+ * little-endian int64) and the n doubles. */
+int mda_state_write_binary(mda_ctx* ctx, const char* path, const double* v, int64_t nx,
+                           int64_t ny, int64_t nlev);
+int mda_state_read_binary(mda_ctx* ctx, const char* path, double* out, int64_t capacity,
+                          int64_t* nx, int64_t* ny, int64_t* nlev);
+/* saveEnsemble (EnKF.hpp:261-282, LETKF.hpp:125-142): <base>_mean.<format>
+ * (mean in Ensemble::RecomputeMean order) and <base>_member_<i>.<format>
+ * in the text format above (ny*nlev rows of nx), each with a .f64 sidecar
+ * beside it when sidecar != 0.  Host or device ensemble. */
+int mda_ensemble_save(mda_ctx* ctx, const mda_ensemble* ens, const char* base,
+                      const char* format, int32_t sidecar);
 
 /* ---- Learned inflation (no counterpart in the reference) -------------------------
  * Cycled twin experiments and a per-column MLP that predicts the LETKF

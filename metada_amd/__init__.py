@@ -6,10 +6,9 @@ analysis interface on top of it.  Importing it does not touch the GPU.
 """
 from . import _lib
 from .analysis import (ColumnGeometry, Config, ConfigValue, Context, EnKF, Ensemble, ETKF,
-                       LETKF, ObsOperator, Observation, default_context, idw4_stencil,
-                       letkf_batch, metrics, read_simple_state)
+                       LETKF, ObsOperator, Observation, default_context, letkf_batch,
+                       metrics, read_simple_state, read_state_sidecar, write_state_sidecar)
 
 __all__ = ["ColumnGeometry", "Config", "ConfigValue", "Context", "EnKF", "Ensemble", "ETKF",
-           "LETKF", "ObsOperator", "Observation", "default_context", "idw4_stencil", "letkf_batch",
-           "metrics",
-           "read_simple_state", "_lib"]
+           "LETKF", "ObsOperator", "Observation", "default_context", "letkf_batch", "metrics",
+           "read_simple_state", "read_state_sidecar", "write_state_sidecar", "_lib"]

@@ -129,6 +129,22 @@ SIGNATURES = {
                                                C.POINTER(Ensemble), C.POINTER(Geometry),
                                                C.POINTER(Obs), C.POINTER(Hop), C.c_void_p,
                                                C.c_void_p, _i32]),
+    "mda_last_io_error": (C.c_char_p, []),
+    "mda_hop_idw4_build": (C.c_int, [C.c_void_p, C.POINTER(Obs), _i32, C.POINTER(Geometry), _i64,
+                                     _i64, _i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     _i32]),
+    "mda_obs_read_grid_file": (C.c_int, [C.c_void_p, C.c_char_p, _d, _d, _i32,
+                                         C.POINTER(C.c_void_p)]),
+    "mda_obs_set_view": (C.c_int, [C.c_void_p, C.POINTER(Obs), C.POINTER(_i32)]),
+    "mda_obs_set_free": (None, [C.c_void_p]),
+    "mda_state_read_text": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, _i64,
+                                      C.POINTER(_i64)]),
+    "mda_state_write_text": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, _i64, _i64]),
+    "mda_state_write_binary": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, _i64, _i64, _i64]),
+    "mda_state_read_binary": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, _i64,
+                                        C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "mda_ensemble_save": (C.c_int, [C.c_void_p, C.POINTER(Ensemble), C.c_char_p, C.c_char_p,
+                                    _i32]),
     "mda_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_void_p, _i32]),
     "mda_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i64,
                                     C.POINTER(_d), C.c_void_p, _i32]),

@@ -256,49 +256,32 @@ class Observation:
     @classmethod
     def from_grid_file(cls, path: str, error: float, missing_value: float = -999.0,
                        coordinate: str = "grid") -> "Observation":
-        """GridObservation::loadFromFile (GridObservation.hpp:401-476): the
-        'Time Z Y X ZNU XLONG_U XLAT_U Value' text format; values equal to
-        missing_value are dropped; R diagonal = error^2."""
-        if not os.path.exists(path):
-            raise RuntimeError(f"Could not open observation file: {path}")
-        header = started = False
-        vals, a, b, c = [], [], [], []
-        with open(path) as f:
-            for line in f:
-                line = line.rstrip("\n")
-                if not line:
-                    continue
-                if "Time" in line and "XLONG_U" in line:
-                    header = True
-                    continue
-                if "---" in line and line.strip(" -\t") == "":
-                    if header:
-                        started = True
-                    continue
-                if not started:
-                    continue
-                parts = line.split()
-                if len(parts) < 8:
-                    continue
-                try:
-                    _t, z, y, x = (int(v) for v in parts[:4])
-                    znu, xlon, xlat, val = (float(v) for v in parts[4:8])
-                except ValueError:
-                    continue
-                if val == missing_value:
-                    continue
-                vals.append(val)
-                if coordinate == "grid":
-                    a.append(float(x)); b.append(float(y)); c.append(float(z))
-                elif coordinate == "geographic":
-                    a.append(xlat); b.append(xlon); c.append(znu)
-                else:
-                    raise RuntimeError(f"Unsupported coordinate system: {coordinate}")
-        if not vals:
-            raise RuntimeError(f"No valid observations loaded from file: {path}")
-        var = np.full(len(vals), float(error) * float(error))
-        return cls(np.array(vals), var, np.array(a), np.array(b), np.array(c),
-                   L.COORD_GRID if coordinate == "grid" else L.COORD_GEOGRAPHIC)
+        """GridObservation::loadFromFile (GridObservation.hpp:401-476) through
+        the library (mda_obs_read_grid_file): the 'Time Z Y X ZNU XLONG_U
+        XLAT_U Value' text format; values equal to missing_value are dropped;
+        R diagonal = error^2, with error as given (the reference's config layer
+        hands it over narrowed through float, GridObservation.hpp:85-86; pass
+        Config(...).Get("error").asFloat() for that)."""
+        coords = {"grid": L.COORD_GRID, "geographic": L.COORD_GEOGRAPHIC}
+        if coordinate not in coords:
+            raise RuntimeError(f"Unsupported coordinate system: {coordinate}")
+        lib = L.load()
+        h = C.c_void_p()
+        rc = lib.mda_obs_read_grid_file(None, os.fsencode(path), float(error),
+                                        float(missing_value), coords[coordinate], C.byref(h))
+        if rc != L.MDA_OK:
+            raise RuntimeError(lib.mda_last_io_error().decode())
+        try:
+            v = L.Obs()
+            cs = C.c_int32()
+            lib.mda_obs_set_view(h, C.byref(v), C.byref(cs))
+            p = v.p
+            arr = lambda ptr: np.ctypeslib.as_array(
+                C.cast(ptr, C.POINTER(C.c_double)), (p,)).copy()
+            return cls(arr(v.value), arr(v.variance), arr(v.loc_a), arr(v.loc_b), arr(v.loc_c),
+                       cs.value)
+        finally:
+            lib.mda_obs_set_free(h)
 
 
 class ObsOperator:
@@ -331,19 +314,30 @@ class ObsOperator:
         return out
 
     @classmethod
-    def identity_idw4(cls, obs: Observation, nx: int, ny: int, nz: int = 1) -> "ObsOperator":
-        """IdentityObsOperator (IdentityObsOperator.hpp:154-180,594-676) for
-        grid-coordinate observations: the four neighbours find4NearestGridPoints
-        selects, weights 1/dist (1e12 on an exact hit), normalised."""
-        rp, cols, ws = [0], [], []
-        for x, y, z in zip(np.asarray(obs.loc_a), np.asarray(obs.loc_b),
-                           np.asarray(obs.loc_c if obs.loc_c is not None
-                                      else np.zeros(obs.size()))):
-            idx, w = idw4_stencil(nx, ny, nz, float(x), float(y), float(z))
-            cols += idx
-            ws += list(np.asarray(w) / np.sum(w))
-            rp.append(len(cols))
-        return cls(np.array(rp, np.int64), np.array(cols, np.int64), np.array(ws))
+    def identity_idw4(cls, obs: Observation, nx: int, ny: int, nz: int = 1,
+                      geometry: "ColumnGeometry" = None, vcoord=None,
+                      ctx: Optional[Context] = None) -> "ObsOperator":
+        """IdentityObsOperator (IdentityObsOperator.hpp:154-281,483-676) as a
+        CSR stencil built on the device (mda_hop_idw4_build): the four
+        neighbours find4NearestGridPoints selects, weights 1/dist (1e12 on an
+        exact hit) normalised; GEOGRAPHIC observations mapped to the nearest
+        grid point of `geometry` (nearest level of vcoord), or to (0, 0, 0)
+        without geographic column coordinates; invalid observations (+inf
+        variance) get empty rows.  The stencil comes back to the host."""
+        ctx = ctx or default_context()
+        os_, ok = _obs_struct(obs)
+        geom = geometry or ColumnGeometry(L.COORD_GRID)
+        gs, gk = geom.struct()
+        p = obs.size()
+        rp = np.zeros(p + 1, np.int64)
+        col = np.zeros(4 * p, np.int64)
+        w = np.zeros(4 * p)
+        vp, _, vk = _buf(vcoord)
+        ctx.check(ctx.lib.mda_hop_idw4_build(ctx.h, C.byref(os_), obs.coord, C.byref(gs), nx, ny,
+                                             nz, vp, rp.ctypes.data, col.ctypes.data,
+                                             w.ctypes.data, 0))
+        nnz = int(rp[-1])
+        return cls(rp, col[:nnz].copy(), w[:nnz].copy())
 
     @classmethod
     def trilinear_latlon(cls, lat, lon, lev, nlon: int, nlat: int, nlev: int) -> "ObsOperator":
@@ -371,25 +365,6 @@ class ObsOperator:
                     t += 1
         rp = np.arange(0, 8 * p + 1, 8, dtype=np.int64)
         return cls(rp, cols.ravel(), ws.ravel())
-
-
-def idw4_stencil(nx, ny, nz, x, y, z):
-    """find4NearestGridPoints + IDW weights (IdentityObsOperator.hpp:594-676)."""
-    x = max(0.0, min(nx - 1.0, x)); y = max(0.0, min(ny - 1.0, y)); z = max(0.0, min(nz - 1.0, z))
-    i0, j0, k0 = int(math.floor(x)), int(math.floor(y)), int(math.floor(z))
-    i1, j1, k1 = min(i0 + 1, nx - 1), min(j0 + 1, ny - 1), min(k0 + 1, nz - 1)
-    if nz == 1:
-        corners = [(i0, j0, 0), (i1, j0, 0), (i0, j1, 0), (i1, j1, 0)]
-        ds = [math.hypot(x - i, y - j) for i, j, _ in corners]
-        chosen = list(zip(ds, corners))
-    else:
-        corners = [(i0, j0, k0), (i1, j0, k0), (i0, j1, k0), (i1, j1, k0),
-                   (i0, j0, k1), (i1, j0, k1), (i0, j1, k1), (i1, j1, k1)]
-        ds = [math.sqrt((x - i) ** 2 + (y - j) ** 2 + (z - kk) ** 2) for i, j, kk in corners]
-        chosen = sorted(zip(ds, corners), key=lambda t: t[0])[:4]
-    idx = [kk * nx * ny + j * nx + i for _, (i, j, kk) in chosen]
-    w = [1e12 if d == 0.0 else 1.0 / d for d, _ in chosen]
-    return idx, w
 
 
 @dataclass
@@ -431,15 +406,16 @@ def _obs_struct(obs: Observation):
 # ---------------------------------------------------------------------------
 # algorithms
 # ---------------------------------------------------------------------------
-def _write_state(path: str, values: np.ndarray, nx: int, ny: int):
-    """SimpleState::saveToFile (SimpleState.hpp:162-188): y rows of x columns,
-    fixed 6 decimals, width 12, single-space separated."""
+def _write_state(path: str, values: np.ndarray, nx: int, nrows: int):
+    """SimpleState::saveToFile (SimpleState.hpp:162-188) through the library:
+    nrows rows of nx values, '%12.6f' separated by one space."""
     d = os.path.dirname(path)
     if d:
         os.makedirs(d, exist_ok=True)
-    with open(path, "w") as f:
-        for y in range(ny):
-            f.write(" ".join(f"{values[y * nx + x]:12.6f}" for x in range(nx)) + "\n")
+    v = np.ascontiguousarray(values, np.float64)
+    lib = L.load()
+    if lib.mda_state_write_text(None, os.fsencode(path), v.ctypes.data, nx, nrows) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
 
 
 class _AlgorithmBase:
@@ -450,13 +426,17 @@ class _AlgorithmBase:
         self.output_base_file = "analysis"
         self.format = "txt"
 
-    def saveEnsemble(self):
-        X = self.ensemble.to_host()
-        nx, ny = self.ensemble.nx, self.ensemble.ny * self.ensemble.nlev
-        mean = X.sum(axis=0) * (1.0 / X.shape[0])
-        _write_state(f"{self.output_base_file}_mean.{self.format}", mean, nx, ny)
-        for i in range(X.shape[0]):
-            _write_state(f"{self.output_base_file}_member_{i}.{self.format}", X[i], nx, ny)
+    def saveEnsemble(self, sidecar: bool = False):
+        """saveEnsemble (EnKF.hpp:261-282): <base>_mean.<format> and
+        <base>_member_<i>.<format> in the 6-decimal text format, plus FP64-exact
+        .f64 sidecars with sidecar=True (mda_ensemble_save)."""
+        d = os.path.dirname(self.output_base_file)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        es, ek = self.ensemble.struct()
+        self.ctx.check(self.ctx.lib.mda_ensemble_save(self.ctx.h, C.byref(es),
+                                                      os.fsencode(self.output_base_file),
+                                                      self.format.encode(), int(sidecar)))
 
 
 class EnKF(_AlgorithmBase):
@@ -601,12 +581,42 @@ def metrics(ensemble: Ensemble, truth, ctx: Optional[Context] = None) -> dict:
 
 
 def read_simple_state(path: str) -> np.ndarray:
-    """SimpleState::readFromFile (SimpleState.hpp:248-263): all whitespace-
-    separated numbers in file order."""
-    if not os.path.exists(path):
-        raise RuntimeError(f"Cannot open state file: {path}")
-    with open(path) as f:
-        return np.array([float(v) for v in f.read().split()])
+    """SimpleState::readFromFile (SimpleState.hpp:248-263) through the
+    library: all whitespace-separated numbers in file order."""
+    lib = L.load()
+    n = C.c_int64(0)
+    if lib.mda_state_read_text(None, os.fsencode(path), None, 0, C.byref(n)) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
+    out = np.empty(n.value)
+    if lib.mda_state_read_text(None, os.fsencode(path), out.ctypes.data, n.value,
+                               C.byref(n)) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
+    return out
+
+
+def read_state_sidecar(path: str):
+    """An FP64-exact state sidecar (mda_state_read_binary): (values, nx, ny, nlev)."""
+    lib = L.load()
+    nx, ny, nl = C.c_int64(), C.c_int64(), C.c_int64()
+    if lib.mda_state_read_binary(None, os.fsencode(path), None, 0, C.byref(nx), C.byref(ny),
+                                 C.byref(nl)) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
+    out = np.empty(nx.value * ny.value * nl.value)
+    if lib.mda_state_read_binary(None, os.fsencode(path), out.ctypes.data, out.size,
+                                 C.byref(nx), C.byref(ny), C.byref(nl)) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
+    return out, nx.value, ny.value, nl.value
+
+
+def write_state_sidecar(path: str, values, nx: int, ny: int, nlev: int = 1):
+    """Writes an FP64-exact state sidecar (mda_state_write_binary)."""
+    lib = L.load()
+    v = np.ascontiguousarray(values, np.float64)
+    if v.size != nx * ny * nlev:
+        raise ValueError("sidecar values do not match nx * ny * nlev")
+    if lib.mda_state_write_binary(None, os.fsencode(path), v.ctypes.data, nx, ny,
+                                  nlev) != L.MDA_OK:
+        raise RuntimeError(lib.mda_last_io_error().decode())
 
 
 def letkf_batch(filters, flags=L.BATCH_GRAPH | L.BATCH_QUEUE | L.BATCH_AUTOTUNE):

@@ -168,6 +168,11 @@ void launch_check_obs(hipStream_t s, const dev_obs& o, int* status) {
   MDA_LAUNCH_CHECK();
 }
 
+// Rethrows the failure of a nested entry point (its message is in ctx->err).
+void check_rc(mda_ctx* c, int rc) {
+  if (rc != MDA_OK) throw mda_error(rc, c->err);
+}
+
 // The analysis's status word as an error (after the stream has drained).
 void throw_status(int st, const char* what) {
   if (st & MDA_ST_NONFINITE_OBS)
@@ -244,7 +249,7 @@ int mda_ctx_destroy(mda_ctx* c) {
   return MDA_OK;
 }
 
-const char* mda_last_error(const mda_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* mda_last_error(const mda_ctx* c) { return c ? c->err.c_str() : mda_last_io_error(); }
 
 int mda_ctx_synchronize(mda_ctx* c) {
   return guarded(c, [&] { MDA_HIP(hipStreamSynchronize(c->stream)); });
@@ -1119,6 +1124,72 @@ int mda_letkf_analyse_batch(mda_ctx* c, int32_t nb, const mda_letkf_params* prm,
       info->ms_total = ms;
       info->graph_replayed = replayed;
       info->n_graphs = (int32_t)c->graphs.size();
+    }
+  });
+}
+
+int mda_hop_idw4_build(mda_ctx* c, const mda_obs* obs, int32_t obs_coord,
+                       const mda_geometry* geom, int64_t nx, int64_t ny, int64_t nlev,
+                       const double* vcoord, int64_t* row_ptr, int64_t* col, double* weight,
+                       int32_t out_on_device) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(obs && row_ptr && col && weight, "observations and CSR outputs are required");
+    MDA_REQUIRE(nx > 0 && ny > 0 && nlev > 0, "grid dimensions must be positive");
+    MDA_REQUIRE(obs_coord == MDA_COORD_GRID || obs_coord == MDA_COORD_GEOGRAPHIC,
+                "IDW4 observations are GRID or GEOGRAPHIC");
+    dev_obs o = stage_obs(c, obs, true);
+    const int64_t p = o.p, ncol = nx * ny;
+    const double *clat = nullptr, *clon = nullptr, *vc = nullptr;
+    if (obs_coord == MDA_COORD_GEOGRAPHIC && geom && geom->coord == MDA_COORD_GEOGRAPHIC) {
+      MDA_REQUIRE(geom->col_a && geom->col_b, "GEOGRAPHIC geometry needs column coordinates");
+      clat = to_device(c, "hop_clat", geom->col_a, ncol, geom->on_device != 0);
+      clon = to_device(c, "hop_clon", geom->col_b, ncol, geom->on_device != 0);
+      vc = to_device(c, "hop_vcoord", vcoord, nlev, geom->on_device != 0);
+    }
+    // rows of 4 taps, none for an invalid observation (variance +inf)
+    std::vector<double> hv(p);
+    d2h(c, hv.data(), o.var, sizeof(double) * p);
+    std::vector<int64_t> rp(p + 1, 0);
+    for (int64_t t = 0; t < p; ++t) rp[t + 1] = rp[t] + (std::isinf(hv[t]) ? 0 : 4);
+    const int64_t nnz = rp[p];
+    int64_t* drp = out_on_device ? row_ptr : c->get<int64_t>("idw_rp", p + 1);
+    int64_t* dcol = out_on_device ? col : c->get<int64_t>("idw_col", nnz);
+    double* dw = out_on_device ? weight : c->get<double>("idw_w", nnz);
+    h2d(c, drp, rp.data(), sizeof(int64_t) * (p + 1));
+    launch_idw4_stencil(c->stream, p, obs_coord, o.a, o.b, o.c, o.var, nx, ny, nlev, clat, clon,
+                        vc, drp, dcol, dw);
+    if (!out_on_device) {
+      std::memcpy(row_ptr, rp.data(), sizeof(int64_t) * (p + 1));
+      d2h(c, col, dcol, sizeof(int64_t) * nnz);
+      d2h(c, weight, dw, sizeof(double) * nnz);
+    }
+    MDA_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mda_ensemble_save(mda_ctx* c, const mda_ensemble* ens, const char* base, const char* format,
+                      int32_t sidecar) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(ens && base && format, "ensemble, base name and format are required");
+    dev_ensemble d = stage_ensemble(c, ens);
+    // Ensemble::RecomputeMean order (Ensemble.hpp:105-114) on the device
+    double* dm = c->get<double>("save_mean", d.n);
+    launch_mean_pert(c->stream, d.X, d.k, d.n, 1.0, dm, nullptr);
+    std::vector<double> host(d.n);
+    const int64_t nrows = ens->ny * ens->nlev;
+    auto write = [&](const std::string& stem) {
+      const std::string txt = stem + "." + format;
+      check_rc(c, mda_state_write_text(c, txt.c_str(), host.data(), ens->nx, nrows));
+      if (sidecar) {
+        const std::string bin = stem + ".f64";
+        check_rc(c, mda_state_write_binary(c, bin.c_str(), host.data(), ens->nx, ens->ny, ens->nlev));
+      }
+    };
+    d2h(c, host.data(), dm, sizeof(double) * d.n);
+    write(std::string(base) + "_mean");
+    for (int j = 0; j < d.k; ++j) {
+      d2h(c, host.data(), d.X + (size_t)j * d.n, sizeof(double) * d.n);
+      write(std::string(base) + "_member_" + std::to_string(j));
     }
   });
 }

@@ -363,6 +363,14 @@ struct etkf_bufs {
 void etkf_run(mda_ctx* c, double inflation, mda_ensemble* ens, const mda_obs* obs,
               const mda_hop* hop, const etkf_bufs* bufs);
 
+// IDW4 observation operator on the device (kernels_hop.hip): 4 taps per
+// row at row_ptr[o] (rows of invalid observations are empty and skipped)
+void launch_idw4_stencil(hipStream_t s, int64_t p, int coord, const double* oa, const double* ob,
+                         const double* oc, const double* var, int64_t nx, int64_t ny,
+                         int64_t nlev, const double* clat, const double* clon,
+                         const double* vcoord, const int64_t* row_ptr, int64_t* col,
+                         double* weight);
+
 // dense p x p algebra of the global filters (p <= MDA_DENSE_MAX)
 constexpr int64_t MDA_DENSE_MAX = 4096;
 struct dense_extremes {

@@ -80,20 +80,6 @@ def test_tutorial_readers():
     assert np.all(obs.variances == 0.1 * 0.1)
 
 
-def test_idw4_csr_matches_oracle(orc):
-    from metada_amd import ObsOperator, Observation
-    rng = np.random.default_rng(0)
-    nx, ny, nz = 9, 7, 3
-    st = rng.standard_normal(nx * ny * nz)
-    pts = rng.uniform(0, 1, (25, 3)) * [nx - 1, ny - 1, nz - 1]
-    pts[:5] = np.round(pts[:5])
-    ob = Observation(np.zeros(25), np.ones(25), pts[:, 0], pts[:, 1], pts[:, 2])
-    H = ObsOperator.identity_idw4(ob, nx, ny, nz)
-    hx = H.apply_host(st[None, :])[0]
-    want = [orc.idw4(st, nx, ny, nz, *p) for p in pts]
-    np.testing.assert_allclose(hx, want, rtol=1e-13, atol=1e-13)
-
-
 def test_trilinear_reproduces_affine_fields():
     from metada_amd import ObsOperator
     nlon, nlat, nlev = 16, 8, 4
