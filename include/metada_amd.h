@@ -132,7 +132,7 @@ typedef struct { /* EnKF::AnalysisResults, EnKF.hpp:84-96 */
   double analysis_spread;
   double max_kalman_gain;
   double min_kalman_gain;
-  double condition_number;        /* NaN when not computed (see DESIGN) */
+  double condition_number;        /* +inf with an invalid obs; NaN beyond p = 4096 */
   int32_t ensemble_size;
   int64_t observation_count;
   int32_t inflation_method;
@@ -145,6 +145,62 @@ typedef struct { /* EnKF::AnalysisResults, EnKF.hpp:84-96 */
 int mda_enkf_analyse(mda_ctx* ctx, double inflation, int32_t inflation_method,
                      mda_ensemble* ens, const mda_obs* obs, const mda_hop* hop,
                      const double* Z, int32_t z_on_device, mda_enkf_diag* diag);
+
+/* ---- LWEnKF (LWEnKF.hpp:207-334) ----------------------------------------------- */
+enum { /* localization_function (LWEnKF.hpp:148-162) */
+  MDA_LW_LOC_GAUSSIAN = 0, MDA_LW_LOC_EXPONENTIAL = 1, MDA_LW_LOC_CUTOFF = 2,
+  MDA_LW_LOC_GASPARI_COHN = 3
+};
+enum { /* what "gaspari_cohn" evaluates */
+  MDA_LW_GC_REFERENCE = 0, /* the reference's polynomial (LWEnKF.hpp:627-636): 4 at 0 */
+  MDA_LW_GC_STANDARD = 1   /* Gaspari & Cohn (1999) eq. (4.10) in nd = dist / radius */
+};
+enum { /* weighting_scheme (LWEnKF.hpp:164-178) */
+  MDA_LW_W_UNIFORM = 0, MDA_LW_W_ADAPTIVE = 1, MDA_LW_W_INVERSE_VAR = 2,
+  MDA_LW_W_LIKELIHOOD = 3
+};
+
+typedef struct {
+  double inflation;           /* X' *= sqrt(inflation) for every method (LWEnKF.hpp:648-668) */
+  double localization_radius; /* in the index distance |i - j| / dim (LWEnKF.hpp:554-598) */
+  int32_t inflation_method;   /* MDA_INFL_*: reported only */
+  int32_t localization;       /* MDA_LW_LOC_* */
+  int32_t gc_form;            /* MDA_LW_GC_* */
+  int32_t weighting;          /* MDA_LW_W_* */
+} mda_lwenkf_params;
+
+typedef struct { /* LWEnKF::AnalysisResults, LWEnKF.hpp:66-84 */
+  double innovation_norm;
+  double analysis_increment_norm; /* never assigned by the reference: 0 */
+  double background_spread;
+  double analysis_spread;
+  double max_kalman_gain;
+  double min_kalman_gain;
+  double condition_number; /* sigma_max / sigma_min of S */
+  double localization_radius;
+  double max_weight;
+  double min_weight;
+  double weight_variance;
+  int32_t ensemble_size;
+  int64_t observation_count;
+  int32_t inflation_method;
+  int32_t localization;
+  int32_t weighting;
+  double inflation_factor;
+} mda_lwenkf_diag;
+
+/* Local weighted EnKF: member weights w, S = L_o o (sum_i w_i y'_i y'_i^T) + R,
+ * K = L_g o (X'_rho Y'^T S^-1 / (k-1)), x_i <- xbar + X'_rho,i + K (yo + sqrt(R) z_i
+ * - H x_i), with L_o and L_g the reference's index-distance tapers.  Z is the
+ * N(0,1) draw matrix the reference takes from std::random_device
+ * (LWEnKF.hpp:690-706), [p][k] row-major.  weights (k values, host) receives
+ * w when not NULL.  2 <= k <= 128, 1 <= p <= 4096.  Errors: MDA_E_ARG for an
+ * invalid observation (+inf variance: the reference's S^-1 would be NaN),
+ * MDA_E_NUMERIC for non-finite input, a singular S or likelihood weights that
+ * all underflow (the reference would divide 0 by 0). */
+int mda_lwenkf_analyse(mda_ctx* ctx, const mda_lwenkf_params* params, mda_ensemble* ens,
+                       const mda_obs* obs, const mda_hop* hop, const double* Z,
+                       int32_t z_on_device, double* weights, mda_lwenkf_diag* diag);
 
 /* ---- ETKF (ETKF.hpp:86-178) ---------------------------------------------------- */
 int mda_etkf_analyse(mda_ctx* ctx, double inflation, mda_ensemble* ens, const mda_obs* obs,

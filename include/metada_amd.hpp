@@ -5,6 +5,9 @@
 //   metada_amd::LETKF<Tag>   mirrors  LETKF<BackendTag>  (LETKF.hpp:47-142)
 //   metada_amd::ETKF<Tag>    mirrors  ETKF<BackendTag>   (ETKF.hpp:84-200)
 //   metada_amd::EnKF<Tag>    mirrors  EnKF<BackendTag>   (EnKF.hpp:84-301)
+//   metada_amd::LWEnKF<Tag>  mirrors  LWEnKF<BackendTag> (LWEnKF.hpp:56-334)
+//   metada_amd::Lorenz96Model is a ModelBackend for Model<BackendTag>
+//                            (ModelConcepts.hpp:51-71, Model.hpp:57-177)
 //
 // Same constructor arguments (non-owning references; the analysis config,
 // or for EnKF the root config whose "analysis" subsection it reads), same
@@ -389,6 +392,244 @@ class EnKF {
   int32_t method_ = MDA_INFL_MULTIPLICATIVE;
   std::vector<double> Z_;
   mda_enkf_diag diag_{};
+};
+
+// LWEnKF<BackendTag>::AnalysisResults (LWEnKF.hpp:66-84)
+struct LWAnalysisResults {
+  double innovation_norm;
+  double analysis_increment_norm;
+  double background_spread;
+  double analysis_spread;
+  double max_kalman_gain;
+  double min_kalman_gain;
+  double condition_number;
+  double localization_radius;
+  std::string localization_function;
+  std::string weighting_scheme;
+  double max_weight;
+  double min_weight;
+  double weight_variance;
+  int ensemble_size;
+  int observation_count;
+  std::string inflation_method;
+  double inflation_factor;
+};
+
+// LWEnKF<BackendTag> (LWEnKF.hpp:103-334): the root config's "analysis"
+// subsection with inflation, localization_radius, output_base_file, format,
+// inflation_method, localization_function and weighting_scheme; unknown
+// names fall back to multiplicative / gaussian / uniform (:109-145).  The
+// N(0,1) draws Z [p][k] are injected (the reference draws them from
+// std::random_device, :690-706).  `standard_gc` replaces the reference's
+// "gaspari_cohn" polynomial by Gaspari & Cohn (1999) eq. (4.10).
+template <class BackendTag>
+class LWEnKF {
+ public:
+  template <class Ens, class Obs, class ObsOp, class Cfg>
+  LWEnKF(Ens& ensemble, Obs& obs, const ObsOp& obs_op, const Cfg& root_config,
+         std::vector<double> Z, bool standard_gc = false,
+         Context& ctx = Context::thread_default())
+      : ctx_(ctx), Z_(std::move(Z)) {
+    const auto ac = root_config.GetSubsection("analysis");
+    prm_.inflation = ac.Get("inflation").asFloat();
+    prm_.localization_radius = ac.Get("localization_radius").asFloat();
+    output_base_file_ = ac.Get("output_base_file").asString();
+    format_ = ac.Get("format").asString();
+    const std::string m = ac.Get("inflation_method").asString();
+    if (m == "additive") {
+      prm_.inflation_method = MDA_INFL_ADDITIVE; method_name_ = m;
+    } else if (m == "relaxation") {
+      prm_.inflation_method = MDA_INFL_RELAXATION; method_name_ = m;
+    } else {
+      prm_.inflation_method = MDA_INFL_MULTIPLICATIVE; method_name_ = "multiplicative";
+    }
+    const std::string f = ac.Get("localization_function").asString();
+    if (f == "exponential") {
+      prm_.localization = MDA_LW_LOC_EXPONENTIAL; loc_name_ = f;
+    } else if (f == "cutoff") {
+      prm_.localization = MDA_LW_LOC_CUTOFF; loc_name_ = f;
+    } else if (f == "gaspari_cohn") {
+      prm_.localization = MDA_LW_LOC_GASPARI_COHN; loc_name_ = f;
+    } else {
+      prm_.localization = MDA_LW_LOC_GAUSSIAN; loc_name_ = "gaussian";
+    }
+    const std::string w = ac.Get("weighting_scheme").asString();
+    if (w == "adaptive") {
+      prm_.weighting = MDA_LW_W_ADAPTIVE; weight_name_ = w;
+    } else if (w == "inverse_var") {
+      prm_.weighting = MDA_LW_W_INVERSE_VAR; weight_name_ = w;
+    } else if (w == "likelihood") {
+      prm_.weighting = MDA_LW_W_LIKELIHOOD; weight_name_ = w;
+    } else {
+      prm_.weighting = MDA_LW_W_UNIFORM; weight_name_ = "uniform";
+    }
+    prm_.gc_form = standard_gc ? MDA_LW_GC_STANDARD : MDA_LW_GC_REFERENCE;
+    run_ = [&ensemble, &obs, &obs_op, this] {
+      this->analyse(ensemble, obs, obs_op);
+      ensemble.RecomputeMean();  // LWEnKF.hpp:303-304
+      ensemble.RecomputePerturbations();
+    };
+    save_ = [&ensemble, this] { detail::save_ensemble(ensemble, output_base_file_, format_); };
+  }
+  void Analyse() { run_(); }
+  void saveEnsemble() const { save_(); }
+  const std::vector<double>& weights() const { return weights_; }
+  LWAnalysisResults getAnalysisResults() const {
+    LWAnalysisResults r;
+    r.innovation_norm = diag_.innovation_norm;
+    r.analysis_increment_norm = diag_.analysis_increment_norm;
+    r.background_spread = diag_.background_spread;
+    r.analysis_spread = diag_.analysis_spread;
+    r.max_kalman_gain = diag_.max_kalman_gain;
+    r.min_kalman_gain = diag_.min_kalman_gain;
+    r.condition_number = diag_.condition_number;
+    r.localization_radius = prm_.localization_radius;
+    r.localization_function = loc_name_;
+    r.weighting_scheme = weight_name_;
+    r.max_weight = diag_.max_weight;
+    r.min_weight = diag_.min_weight;
+    r.weight_variance = diag_.weight_variance;
+    r.ensemble_size = diag_.ensemble_size;
+    r.observation_count = static_cast<int>(diag_.observation_count);
+    r.inflation_method = method_name_;
+    r.inflation_factor = prm_.inflation;
+    return r;
+  }
+
+ private:
+  template <class Ens, class Obs, class ObsOp>
+  void analyse(Ens& ens, Obs& obs, const ObsOp& op) {
+    detail::Staged s = detail::stage(ens, obs, op);
+    if (static_cast<int64_t>(Z_.size()) != s.p * s.k)
+      throw std::runtime_error("Z must hold p*k standard normal draws");
+    mda_ensemble e = detail::ensemble_view(s, s.n, 1, 1);
+    mda_obs o = detail::obs_view(s);
+    mda_hop h = detail::hx_view(s);
+    weights_.assign(static_cast<size_t>(s.k), 0.0);
+    check(ctx_.get(),
+          mda_lwenkf_analyse(ctx_.get(), &prm_, &e, &o, &h, Z_.data(), 0, weights_.data(),
+                             &diag_),
+          "LWEnKF::Analyse");
+  }
+  std::function<void()> run_, save_;
+  Context& ctx_;
+  mda_lwenkf_params prm_{};
+  std::string output_base_file_, format_, method_name_, loc_name_, weight_name_;
+  std::vector<double> Z_, weights_;
+  mda_lwenkf_diag diag_{};
+};
+
+// Lorenz-96 forecast as a metada model backend: it satisfies the
+// ModelBackendImpl concept (ModelConcepts.hpp:51-71) -- constructed from the
+// config backend, initialize / reset / finalize / isInitialized, run(initial,
+// final) on state backends, no default constructor and no copies -- so
+// BackendTraits<Tag>::ModelBackend = metada_amd::Lorenz96Model puts the
+// library's forecast behind Model<Tag> (Model.hpp:57-177) unchanged.
+//
+// Config keys, each optional (YamlConfig Get / HasKey, YamlConfig.hpp:132-149;
+// reals arrive float-narrowed as every config real does):
+//   forcing    F (8.0, lorenz95.py:11)      dt         time step (0.05)
+//   num_steps  steps per run (1)            scheme     "euler" (the forward
+//              step of lorenz95.py:21-31 with its clipping) or "rk4"
+// run() reads the state backend through getData() / size()
+// (SimpleState.hpp:69,88); runEnsemble() advances every member of an
+// Ensemble in one launch.  4 <= N <= 6144.
+class Lorenz96Model {
+ public:
+  Lorenz96Model() = delete;
+  Lorenz96Model(const Lorenz96Model&) = delete;
+  Lorenz96Model& operator=(const Lorenz96Model&) = delete;
+  Lorenz96Model(Lorenz96Model&& o) noexcept
+      : ctx_(o.ctx_), forcing_(o.forcing_), dt_(o.dt_), steps_(o.steps_),
+        scheme_(o.scheme_), initialized_(o.initialized_) {
+    o.initialized_ = false;
+  }
+  ~Lorenz96Model() = default;
+
+  template <class ConfigBackend>
+  explicit Lorenz96Model(const ConfigBackend& config, Context& ctx = Context::thread_default())
+      : ctx_(&ctx) {
+    initialize(config);
+  }
+
+  template <class ConfigBackend>
+  void initialize(const ConfigBackend& config) {
+    if (initialized_) return;
+    auto real = [&](const char* key, double def) -> double {
+      if (!config.HasKey(key)) return def;
+      const auto v = config.Get(key);
+      return v.isInt() ? static_cast<double>(v.asInt()) : static_cast<double>(v.asFloat());
+    };
+    forcing_ = real("forcing", 8.0);
+    dt_ = real("dt", 0.05);
+    steps_ = config.HasKey("num_steps") ? config.Get("num_steps").asInt() : 1;
+    const std::string s = config.HasKey("scheme") ? config.Get("scheme").asString() : "euler";
+    if (s == "euler") {
+      scheme_ = 1;
+    } else if (s == "rk4") {
+      scheme_ = 0;
+    } else {
+      throw std::runtime_error("Lorenz96Model: unknown scheme '" + s + "' (euler or rk4)");
+    }
+    if (steps_ < 0) throw std::runtime_error("Lorenz96Model: num_steps must be >= 0");
+    if (!(dt_ > 0.0)) throw std::runtime_error("Lorenz96Model: dt must be positive");
+    initialized_ = true;
+  }
+  bool isInitialized() const { return initialized_; }
+  void reset() {
+    if (!initialized_) throw std::runtime_error("Lorenz96Model not initialized");
+  }
+  void finalize() { initialized_ = false; }
+
+  // final = M^num_steps(initial)
+  template <class StateBackend>
+  void run(const StateBackend& initial, StateBackend& final_state) {
+    if (!initialized_) throw std::runtime_error("Lorenz96Model not initialized");
+    const double* in = static_cast<const double*>(initial.getData());
+    double* out = static_cast<double*>(final_state.getData());
+    const size_t n = initial.size();
+    if (final_state.size() != n) throw std::runtime_error("Lorenz96Model: state sizes differ");
+    if (out != in) std::copy(in, in + n, out);
+    advance(out, 1, static_cast<int64_t>(n));
+  }
+
+  // Every member of an Ensemble (Ensemble.hpp: Size, GetMember; State.hpp:
+  // getDataPtr, size) forecast together, in place.
+  template <class Ens>
+  void runEnsemble(Ens& ens) {
+    if (!initialized_) throw std::runtime_error("Lorenz96Model not initialized");
+    const size_t k = ens.Size();
+    if (k == 0) return;
+    const size_t n = ens.GetMember(0).size();
+    std::vector<double> buf(k * n);
+    for (size_t j = 0; j < k; ++j) {
+      auto& m = ens.GetMember(j);
+      if (m.size() != n) throw std::runtime_error("ensemble members differ in size");
+      const double* d = m.template getDataPtr<double>();
+      std::copy(d, d + n, buf.begin() + j * n);
+    }
+    advance(buf.data(), static_cast<int64_t>(k), static_cast<int64_t>(n));
+    for (size_t j = 0; j < k; ++j)
+      std::copy(buf.begin() + j * n, buf.begin() + (j + 1) * n,
+                ens.GetMember(j).template getDataPtr<double>());
+  }
+
+  double forcing() const { return forcing_; }
+  double dt() const { return dt_; }
+  int numSteps() const { return steps_; }
+
+ private:
+  void advance(double* x, int64_t nstates, int64_t n) {
+    if (steps_ == 0) return;
+    check(ctx_->get(),
+          mda_l96_forecast(ctx_->get(), x, nstates, n, forcing_, dt_, steps_, scheme_, 0),
+          "Lorenz96Model::run");
+  }
+  Context* ctx_;
+  double forcing_ = 8.0, dt_ = 0.05;
+  int steps_ = 1;
+  int32_t scheme_ = 1;
+  bool initialized_ = false;
 };
 
 }  // namespace metada_amd

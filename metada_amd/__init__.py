@@ -5,10 +5,12 @@ include/metada_amd.h); this package is the host-side mirror of metada's
 analysis interface on top of it.  Importing it does not touch the GPU.
 """
 from . import _lib
+from .model import Lorenz96Model, Model
 from .analysis import (ColumnGeometry, Config, ConfigValue, Context, EnKF, Ensemble, ETKF,
-                       LETKF, ObsOperator, Observation, default_context, letkf_batch,
+                       LETKF, LWEnKF, ObsOperator, Observation, default_context, letkf_batch,
                        metrics, read_simple_state, read_state_sidecar, write_state_sidecar)
 
 __all__ = ["ColumnGeometry", "Config", "ConfigValue", "Context", "EnKF", "Ensemble", "ETKF",
-           "LETKF", "ObsOperator", "Observation", "default_context", "letkf_batch", "metrics",
-           "read_simple_state", "read_state_sidecar", "write_state_sidecar", "_lib"]
+           "LETKF", "LWEnKF", "Lorenz96Model", "Model", "ObsOperator", "Observation", "default_context",
+           "letkf_batch", "metrics", "read_simple_state", "read_state_sidecar",
+           "write_state_sidecar", "_lib"]

@@ -19,6 +19,9 @@ ERRORS = {-1: "MDA_E_ARG", -2: "MDA_E_DEVICE", -3: "MDA_E_NOMEM", -4: "MDA_E_NUM
 COORD_GRID, COORD_GEOGRAPHIC, COORD_CARTESIAN = 0, 1, 2
 HOP_CSR, HOP_HX = 0, 1
 INFL_MULTIPLICATIVE, INFL_ADDITIVE, INFL_RELAXATION = 0, 1, 2
+LW_LOC_GAUSSIAN, LW_LOC_EXPONENTIAL, LW_LOC_CUTOFF, LW_LOC_GASPARI_COHN = 0, 1, 2, 3
+LW_GC_REFERENCE, LW_GC_STANDARD = 0, 1
+LW_W_UNIFORM, LW_W_ADAPTIVE, LW_W_INVERSE_VAR, LW_W_LIKELIHOOD = 0, 1, 2, 3
 LOC_GASPARI_COHN, LOC_CUTOFF = 0, 1
 RW_REAL, RW_IDENTITY = 0, 1
 INFLFORM_PRIOR, INFLFORM_POSTERIOR = 0, 1
@@ -59,6 +62,20 @@ class EnkfDiag(C.Structure):
                 ("min_kalman_gain", _d), ("condition_number", _d), ("ensemble_size", _i32),
                 ("observation_count", _i64), ("inflation_method", _i32),
                 ("inflation_factor", _d)]
+
+
+class LwenkfParams(C.Structure):
+    _fields_ = [("inflation", _d), ("localization_radius", _d), ("inflation_method", _i32),
+                ("localization", _i32), ("gc_form", _i32), ("weighting", _i32)]
+
+
+class LwenkfDiag(C.Structure):
+    _fields_ = [("innovation_norm", _d), ("analysis_increment_norm", _d),
+                ("background_spread", _d), ("analysis_spread", _d), ("max_kalman_gain", _d),
+                ("min_kalman_gain", _d), ("condition_number", _d), ("localization_radius", _d),
+                ("max_weight", _d), ("min_weight", _d), ("weight_variance", _d),
+                ("ensemble_size", _i32), ("observation_count", _i64), ("inflation_method", _i32),
+                ("localization", _i32), ("weighting", _i32), ("inflation_factor", _d)]
 
 
 class LetkfParams(C.Structure):
@@ -106,6 +123,9 @@ SIGNATURES = {
                                       _i32]),
     "mda_enkf_analyse": (C.c_int, [C.c_void_p, _d, _i32, C.POINTER(Ensemble), C.POINTER(Obs),
                                    C.POINTER(Hop), C.c_void_p, _i32, C.POINTER(EnkfDiag)]),
+    "mda_lwenkf_analyse": (C.c_int, [C.c_void_p, C.POINTER(LwenkfParams), C.POINTER(Ensemble),
+                                     C.POINTER(Obs), C.POINTER(Hop), C.c_void_p, _i32,
+                                     C.c_void_p, C.POINTER(LwenkfDiag)]),
     "mda_etkf_analyse": (C.c_int, [C.c_void_p, _d, C.POINTER(Ensemble), C.POINTER(Obs),
                                    C.POINTER(Hop)]),
     "mda_etkf_analyse_noalloc": (C.c_int, [C.c_void_p, _d, C.POINTER(Ensemble), C.POINTER(Obs),

@@ -482,6 +482,82 @@ class EnKF(_AlgorithmBase):
         return d
 
 
+class LWEnKF(_AlgorithmBase):
+    """LWEnKF<Tag> (LWEnKF.hpp:207-334).  Reads the `analysis` subsection of
+    the root config (LWEnKF.hpp:103-145): inflation, localization_radius,
+    inflation_method, localization_function and weighting_scheme, unknown
+    names falling back to multiplicative / gaussian / uniform as the
+    reference's do.  Every inflation method scales X' by sqrt(inflation)
+    (LWEnKF.hpp:648-668).  gc_form "reference" keeps the reference's
+    "gaspari_cohn" polynomial (4 at distance 0); "standard" evaluates Gaspari
+    & Cohn (1999) eq. (4.10) in its place.  The N(0,1) draws the reference
+    takes from std::random_device are passed in (`Z`, shape (p, k)) or drawn
+    from a seeded numpy Generator."""
+
+    METHODS = EnKF.METHODS
+    LOCS = {"gaussian": L.LW_LOC_GAUSSIAN, "exponential": L.LW_LOC_EXPONENTIAL,
+            "cutoff": L.LW_LOC_CUTOFF, "gaspari_cohn": L.LW_LOC_GASPARI_COHN}
+    WEIGHTS = {"uniform": L.LW_W_UNIFORM, "adaptive": L.LW_W_ADAPTIVE,
+               "inverse_var": L.LW_W_INVERSE_VAR, "likelihood": L.LW_W_LIKELIHOOD}
+
+    def __init__(self, ensemble, obs, obs_op, config: Config, ctx=None, seed: int = 0,
+                 gc_form: str = "reference"):
+        super().__init__(ensemble, obs, obs_op, ctx)
+        ac = config.GetSubsection("analysis")
+        self.inflation_factor = ac.Get("inflation").asFloat()
+        self.localization_radius = ac.Get("localization_radius").asFloat()
+        self.output_base_file = ac.Get("output_base_file").v or "analysis"
+        self.format = ac.Get("format").v or "txt"
+        m, f, w = (ac.Get(key).v for key in
+                   ("inflation_method", "localization_function", "weighting_scheme"))
+        self.inflation_method_name = m if m in self.METHODS else "multiplicative"
+        self.localization_function = f if f in self.LOCS else "gaussian"
+        self.weighting_scheme = w if w in self.WEIGHTS else "uniform"
+        if gc_form not in ("reference", "standard"):
+            raise ValueError("gc_form is 'reference' or 'standard'")
+        self.gc_form = gc_form
+        self.rng = np.random.default_rng(seed)
+        self.weights = None
+        self._diag = None
+
+    def params(self) -> "L.LwenkfParams":
+        return L.LwenkfParams(self.inflation_factor, self.localization_radius,
+                              self.METHODS[self.inflation_method_name],
+                              self.LOCS[self.localization_function],
+                              L.LW_GC_STANDARD if self.gc_form == "standard"
+                              else L.LW_GC_REFERENCE,
+                              self.WEIGHTS[self.weighting_scheme])
+
+    def Analyse(self, Z=None):
+        k, p = self.ensemble.Size(), self.obs.size()
+        if Z is None:
+            Z = self.rng.standard_normal((p, k))
+        es, ek = self.ensemble.struct()
+        os_, ok = _obs_struct(self.obs)
+        hs, hk = self.obs_op.struct()
+        zp, zdev, zk = _buf(Z)
+        prm = self.params()
+        w = np.zeros(k)
+        dg = L.LwenkfDiag()
+        self.ctx.check(self.ctx.lib.mda_lwenkf_analyse(
+            self.ctx.h, C.byref(prm), C.byref(es), C.byref(os_), C.byref(hs), zp, zdev,
+            w.ctypes.data, C.byref(dg)))
+        self.weights = w
+        self._diag = dg
+        self.ensemble._mean = None
+
+    def getAnalysisResults(self) -> dict:
+        """LWEnKF::AnalysisResults (LWEnKF.hpp:66-84, 340-362)."""
+        if self._diag is None:
+            raise RuntimeError("Analyse() has not been called")
+        d = {f: getattr(self._diag, f) for f, _ in L.LwenkfDiag._fields_}
+        d["inflation_method"] = self.inflation_method_name
+        d["localization_function"] = self.localization_function
+        d["weighting_scheme"] = self.weighting_scheme
+        del d["localization"], d["weighting"]
+        return d
+
+
 class ETKF(_AlgorithmBase):
     """ETKF<Tag> (ETKF.hpp:86-178); config is the analysis subsection."""
 

@@ -480,6 +480,173 @@ int mda_enkf_analyse(mda_ctx* c, double inflation, int32_t method, mda_ensemble*
 }
 
 // ---------------------------------------------------------------------------
+// LWEnKF (LWEnKF.hpp:207-334).  Member weights are k numbers formed on the
+// host from device reductions in the reference's order (:413-500): the
+// adaptive and inverse-variance forms normalise twice, as the reference does.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::vector<double> lw_weights(int scheme, const std::vector<double>& sumsq,
+                               const std::vector<double>& chi2, int k) {
+  std::vector<double> w(k);
+  if (scheme == MDA_LW_W_ADAPTIVE || scheme == MDA_LW_W_INVERSE_VAR) {
+    double sum_inv = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double m = scheme == MDA_LW_W_ADAPTIVE ? std::sqrt(sumsq[i]) : sumsq[i];
+      w[i] = 1.0 / (m + 1e-8);
+      sum_inv += w[i];
+    }
+    for (int i = 0; i < k; ++i) w[i] /= sum_inv;
+  } else if (scheme == MDA_LW_W_LIKELIHOOD) {
+    for (int i = 0; i < k; ++i) w[i] = std::exp(-0.5 * chi2[i]);
+  } else {
+    for (int i = 0; i < k; ++i) w[i] = 1.0 / k;
+  }
+  double sum = 0.0;
+  for (int i = 0; i < k; ++i) sum += w[i];
+  if (!(sum > 0.0) || !std::isfinite(sum))
+    throw mda_error(MDA_E_NUMERIC,
+                    "LWEnKF: the member weights sum to zero or a non-finite value (likelihood "
+                    "weights underflow when every member is far from the observations)");
+  for (int i = 0; i < k; ++i) w[i] /= sum;
+  return w;
+}
+
+}  // namespace
+
+int mda_lwenkf_analyse(mda_ctx* c, const mda_lwenkf_params* prm, mda_ensemble* ens,
+                       const mda_obs* obs, const mda_hop* hop, const double* Z,
+                       int32_t z_on_device, double* weights, mda_lwenkf_diag* diag) {
+  return guarded(c, [&] {
+    MDA_REQUIRE(prm != nullptr, "LWEnKF parameters are required");
+    MDA_REQUIRE(std::isfinite(prm->inflation) && prm->inflation >= 0.0, "inflation must be >= 0");
+    MDA_REQUIRE(prm->localization_radius > 0.0, "localization radius must be > 0");
+    MDA_REQUIRE(prm->localization >= MDA_LW_LOC_GAUSSIAN &&
+                    prm->localization <= MDA_LW_LOC_GASPARI_COHN,
+                "unknown LWEnKF localization function");
+    MDA_REQUIRE(prm->gc_form == MDA_LW_GC_REFERENCE || prm->gc_form == MDA_LW_GC_STANDARD,
+                "unknown Gaspari-Cohn form");
+    MDA_REQUIRE(prm->weighting >= MDA_LW_W_UNIFORM && prm->weighting <= MDA_LW_W_LIKELIHOOD,
+                "unknown LWEnKF weighting scheme");
+    MDA_REQUIRE(Z != nullptr, "LWEnKF needs the N(0,1) draws Z");
+    dev_ensemble de = stage_ensemble(c, ens);
+    dev_obs o = stage_obs(c, obs, false);
+    dev_hop h = stage_hop(c, hop, de.k, o.p, de.n);
+    const int k = de.k;
+    const int64_t p = o.p, n = de.n;
+    MDA_REQUIRE(k >= 2 && k <= 128, "LWEnKF supports 2 <= k <= 128");
+    MDA_REQUIRE(p <= MDA_DENSE_MAX, "LWEnKF supports p <= 4096 (dense S)");
+    hipStream_t s = c->stream;
+    const double* dZ = to_device(c, "lw_Z", Z, (size_t)p * k, z_on_device != 0);
+    const double scale = std::sqrt(prm->inflation);
+    const double radius = prm->localization_radius;
+    int* st = c->get<int>("lw_status", 2);  // [0] status word, [1] invalid-observation flag
+    launch_zero(s, st, 2 * sizeof(int));
+    launch_check_obs(s, o, st);
+    launch_lw_flag_inf(s, o.var, p, st + 1);
+    // forecast statistics (RecomputeMean / RecomputePerturbations, :215-229)
+    double* mean = c->get<double>("lw_mean", n);
+    launch_mean_pert(s, de.X, k, n, 1.0, mean, nullptr);
+    double* ybar = c->get<double>("lw_ybar", p);
+    double* yp = c->get<double>("lw_yp", (size_t)p * k);
+    double* dd = c->get<double>("lw_d", p);
+    launch_obs_stats(s, de.X, k, n, p, h.row_ptr, h.col, h.w, h.hx, o.yo, ybar, yp, dd, nullptr,
+                     nullptr, nullptr, nullptr, st);
+    int hst[2] = {0, 0};
+    d2h(c, hst, st, sizeof(hst));
+    MDA_HIP(hipStreamSynchronize(s));
+    if (hst[1])
+      throw mda_error(MDA_E_ARG, "LWEnKF: invalid observations (+inf variance) are not supported");
+    throw_status(hst[0] & MDA_ST_SKIP_UPDATE, "LWEnKF");
+    // member weights (:413-430)
+    std::vector<double> sumsq(k, 0.0), chi2(k, 0.0);
+    if (prm->weighting == MDA_LW_W_ADAPTIVE || prm->weighting == MDA_LW_W_INVERSE_VAR) {
+      constexpr int NB = 64;
+      double* part = c->get<double>("lw_part", (size_t)NB * k);
+      launch_lw_member_sumsq(s, de.X, mean, k, n, part, NB);
+      std::vector<double> hp((size_t)NB * k);
+      d2h(c, hp.data(), part, sizeof(double) * NB * k);
+      MDA_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < k; ++i)
+        for (int b = 0; b < NB; ++b) sumsq[i] += hp[(size_t)i * NB + b];
+    } else if (prm->weighting == MDA_LW_W_LIKELIHOOD) {
+      double* dchi = c->get<double>("lw_chi2", k);
+      launch_lw_chi2(s, o.yo, ybar, yp, o.var, k, p, dchi);
+      d2h(c, chi2.data(), dchi, sizeof(double) * k);
+      MDA_HIP(hipStreamSynchronize(s));
+    }
+    const std::vector<double> w = lw_weights(prm->weighting, sumsq, chi2, k);
+    double* dw = c->get<double>("lw_w", k);
+    h2d(c, dw, w.data(), sizeof(double) * k);
+    // perturbed-observation innovations (:286-301) before Y' is overwritten
+    double* D = c->get<double>("lw_D", (size_t)p * k);
+    launch_lw_innovations(s, o.yo, ybar, yp, o.var, dZ, k, p, D);
+    // S = L_o o Pf + R and its transpose (:258-270)
+    double* S = c->get<double>("lw_S", (size_t)p * p);
+    double* St = c->get<double>("lw_St", (size_t)p * p);
+    launch_lw_form_S(s, yp, dw, o.var, k, p, radius, prm->localization, prm->gc_form, S, St);
+    mda_lwenkf_diag dg{};
+    if (diag) {
+      // cond(S) = sigma_max / sigma_min (:280-284); S is symmetric, so its
+      // singular values are the |eigenvalues|
+      const dense_extremes x = dense_sym_extremes(c, S, p);
+      dg.condition_number = x.absmax / x.absmin;
+    }
+    // G^T = S^-T Y' (p x k), in place of Y'
+    int64_t* piv = c->get<int64_t>("lw_piv", 1);
+    double* pval = c->get<double>("lw_pval", 1);
+    dense_gauss_jordan(s, p, k, St, yp, piv, pval, st);
+    double* part2 = c->get<double>("sumsq_part", 1024);
+    if (diag) {
+      launch_dev_sumsq(s, de.X, k, n, scale, part2, 512);
+      dg.background_spread = std::sqrt(host_sum(c, part2, 512) / ((double)n * k));
+      launch_sumsq(s, dd, p, part2, 512);
+      dg.innovation_norm = std::sqrt(host_sum(c, part2, 512));
+    }
+    d2h(c, hst, st, sizeof(int));
+    MDA_HIP(hipStreamSynchronize(s));
+    if (hst[0] & MDA_ST_NOT_SPD)
+      throw mda_error(MDA_E_NUMERIC, "LWEnKF: S = L o Pf + R is singular");
+    // gain and update, fused (:272-301)
+    const size_t nbu = lw_update_blocks(n);
+    double* mm = c->get<double>("lw_mm", 2 * nbu);
+    launch_lw_update(s, de.X, mean, scale, yp, D, k, n, p, radius, prm->localization,
+                     prm->gc_form, mm);
+    std::vector<double> hm(2 * nbu);
+    d2h(c, hm.data(), mm, sizeof(double) * 2 * nbu);
+    MDA_HIP(hipStreamSynchronize(s));
+    double mx = -std::numeric_limits<double>::infinity(), mn = -mx;
+    for (size_t b = 0; b < nbu; ++b) {
+      mx = std::max(mx, hm[2 * b]);
+      mn = std::min(mn, hm[2 * b + 1]);
+    }
+    if (diag) {
+      launch_dev_sumsq(s, de.X, k, n, 1.0, part2, 512);
+      dg.analysis_spread = std::sqrt(host_sum(c, part2, 512) / ((double)n * k));
+      dg.max_kalman_gain = mx;
+      dg.min_kalman_gain = mn;
+      dg.localization_radius = radius;
+      dg.max_weight = *std::max_element(w.begin(), w.end());
+      dg.min_weight = *std::min_element(w.begin(), w.end());
+      const double mw = 1.0 / (double)k;
+      double wv = 0.0;
+      for (double v : w) wv += (v - mw) * (v - mw);
+      dg.weight_variance = wv / (double)k;
+      dg.ensemble_size = k;
+      dg.observation_count = p;
+      dg.inflation_method = prm->inflation_method;
+      dg.localization = prm->localization;
+      dg.weighting = prm->weighting;
+      dg.inflation_factor = prm->inflation;
+      dg.analysis_increment_norm = 0.0;
+    }
+    unstage_ensemble(c, ens, de);
+    if (weights) std::copy(w.begin(), w.end(), weights);
+    if (diag) *diag = dg;
+  });
+}
+
+// ---------------------------------------------------------------------------
 void mda_letkf_params_standard(mda_letkf_params* p, double inflation, double radius) {
   if (!p) return;
   std::memset(p, 0, sizeof(*p));

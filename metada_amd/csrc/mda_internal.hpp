@@ -391,4 +391,21 @@ double dense_spd_condition(mda_ctx* c, const double* yp, const double* var, int 
 void dense_gauss_jordan(hipStream_t s, int64_t p, int64_t m, double* A, double* B,
                         int64_t* piv, double* pval, int* status);
 
+// LWEnKF (kernels_lw.hip)
+void launch_lw_member_sumsq(hipStream_t s, const double* X, const double* mean, int k, int64_t n,
+                            double* partial, int nblocks);
+void launch_lw_chi2(hipStream_t s, const double* yo, const double* ybar, const double* yp_om,
+                    const double* var, int k, int64_t p, double* chi2);
+void launch_lw_form_S(hipStream_t s, const double* yp_om, const double* w, const double* var,
+                      int k, int64_t p, double radius, int fn, int gc_form, double* S,
+                      double* St);
+void launch_lw_innovations(hipStream_t s, const double* yo, const double* ybar,
+                           const double* yp_om, const double* var, const double* Z, int k,
+                           int64_t p, double* D);
+size_t lw_update_blocks(int64_t n);
+void launch_lw_update(hipStream_t s, double* X, const double* mean, double scale,
+                      const double* Gt, const double* D, int k, int64_t n, int64_t p,
+                      double radius, int fn, int gc_form, double* mm);
+void launch_lw_flag_inf(hipStream_t s, const double* var, int64_t p, int* flag);
+
 }  // namespace mda

@@ -510,6 +510,193 @@ done:
 }
 
 /* ======================================================================== */
+/* LWEnKF                                                                     */
+/* ======================================================================== */
+
+/* computeLocalizationFunction / computeGaspariCohnFunction, LWEnKF.hpp:600-636.
+ * The reference's "gaspari_cohn" polynomial is kept as written (it is 4 at
+ * 0); gc_standard = 1 evaluates Gaspari & Cohn (1999) eq. (4.10) instead. */
+double orc_lw_taper(double dist, double radius, int fn, int gc_standard) {
+  double nd = dist / radius;
+  switch (fn) {
+    case ORC_LW_EXPONENTIAL:
+      return exp(-nd);
+    case ORC_LW_CUTOFF:
+      return nd <= 1.0 ? 1.0 : 0.0;
+    case ORC_LW_GASPARI_COHN:
+      if (gc_standard) return orc_gaspari_cohn(nd);
+      if (nd >= 2.0) return 0.0;
+      if (nd >= 1.0) {
+        double z = nd - 1.0;
+        return ((-0.25 * z + 0.5) * z + 0.625) * z + 0.125;
+      } else {
+        double z = nd;
+        return (((-0.25 * z + 0.5) * z + 0.625) * z - 5.0) * z + 4.0;
+      }
+    default:
+      return exp(-0.5 * nd * nd);
+  }
+}
+
+/* LWEnKF::Analyse, LWEnKF.hpp:207-334, with the N(0,1) draws Z [p][k]
+ * injected (they come from std::random_device at LWEnKF.hpp:690-706).
+ *   weights      computeWeights and its helpers (:413-500): uniform 1/k;
+ *                adaptive 1/(||x'_i|| + 1e-8) and inverse_var
+ *                1/(||x'_i||^2 + 1e-8), each normalised twice; likelihood
+ *                exp(-0.5 (yo - Hx_i)^T R^-1 (yo - Hx_i)), normalised once
+ *   Pf           sum_i (w_i y'_i) y'_i^T over un-inflated Y' (:502-513)
+ *   S            Pf o L_o + R, L_o(a,b) = f(|a-b| / p) (:515-533)
+ *   K            (X'_rho Y'^T S^-1) / (k-1) o L_g, L_g(s,b) = f(|s-b| / max(n,p))
+ *                (:272-279, :535-553)
+ *   update       x_i = xbar + (X'_rho,i + K (yo + sqrt(r) z_i - Hx_i)) (:286-301)
+ * Yb is [k][p]; X [k][n] is updated in place. */
+int orc_lwenkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
+                       const double* yo, const double* var, const double* Z, double inflation,
+                       double radius, int fn, int gc_standard, int weighting, double* weights,
+                       double* Kout, orc_lwenkf_diag* diag) {
+  if (k < 2 || n <= 0 || p <= 0 || !(radius > 0.0)) return ORC_E_ARG;
+  size_t nk = (size_t)n * k, pk = (size_t)p * k, pp = (size_t)p * p;
+  double* mean = (double*)malloc(sizeof(double) * n);
+  double* Xp = (double*)malloc(sizeof(double) * nk);
+  double* ybar = (double*)malloc(sizeof(double) * p);
+  double* Yp = (double*)malloc(sizeof(double) * pk);
+  double* d = (double*)malloc(sizeof(double) * p);
+  double* S = (double*)malloc(sizeof(double) * pp);
+  double* Sinv = (double*)malloc(sizeof(double) * pp);
+  double* XY = (double*)malloc(sizeof(double) * (size_t)n * p);
+  double* K = (double*)malloc(sizeof(double) * (size_t)n * p);
+  double* sv = (double*)malloc(sizeof(double) * p);
+  double* di = (double*)malloc(sizeof(double) * p);
+  double* w = (double*)malloc(sizeof(double) * k);
+  int rc = 0;
+  if (!mean || !Xp || !ybar || !Yp || !d || !S || !Sinv || !XY || !K || !sv || !di || !w) {
+    rc = ORC_E_NOMEM;
+    goto done;
+  }
+  /* 1. mean and un-inflated perturbations (:215-229) */
+  orc_ensemble_mean(k, n, X, mean);
+  orc_ensemble_perturbations(k, n, X, mean, 1.0, Xp);
+  /* 2. weights (:413-430) */
+  if (weighting == ORC_LW_ADAPTIVE || weighting == ORC_LW_INVERSE_VAR) {
+    double sum_inv = 0.0;
+    for (int j = 0; j < k; ++j) {
+      double ss = 0.0;
+      for (int64_t i = 0; i < n; ++i) ss += Xp[(size_t)j * n + i] * Xp[(size_t)j * n + i];
+      double m = weighting == ORC_LW_ADAPTIVE ? sqrt(ss) : ss;
+      w[j] = 1.0 / (m + 1e-8);
+      sum_inv += w[j];
+    }
+    for (int j = 0; j < k; ++j) w[j] /= sum_inv;
+  } else if (weighting == ORC_LW_LIKELIHOOD) {
+    for (int j = 0; j < k; ++j) {
+      double chi2 = 0.0;
+      for (int64_t o = 0; o < p; ++o) {
+        double inn = yo[o] - Yb[(size_t)j * p + o];
+        chi2 += inn * (1.0 / var[o]) * inn;
+      }
+      w[j] = exp(-0.5 * chi2);
+    }
+  } else {
+    for (int j = 0; j < k; ++j) w[j] = 1.0 / k;
+  }
+  {
+    double sum = 0.0;
+    for (int j = 0; j < k; ++j) sum += w[j];
+    if (!(sum > 0.0)) { rc = ORC_E_SINGULAR; goto done; }
+    for (int j = 0; j < k; ++j) w[j] /= sum;
+    double mx = w[0], mn = w[0], mw = 1.0 / k, wv = 0.0;
+    for (int j = 0; j < k; ++j) {
+      if (w[j] > mx) mx = w[j];
+      if (w[j] < mn) mn = w[j];
+      wv += (w[j] - mw) * (w[j] - mw);
+    }
+    diag->max_weight = mx;
+    diag->min_weight = mn;
+    diag->weight_variance = wv / k;
+    if (weights) memcpy(weights, w, sizeof(double) * k);
+  }
+  /* 3. inflation of X' only (:648-668) */
+  {
+    double sc = sqrt(inflation), ss = 0.0;
+    for (size_t t = 0; t < nk; ++t) {
+      Xp[t] *= sc;
+      ss += Xp[t] * Xp[t];
+    }
+    diag->background_spread = sqrt(ss / (double)nk);
+  }
+  /* 4-5. observation space, innovation (:240-255) */
+  obs_stats(k, p, Yb, yo, ybar, Yp, d);
+  {
+    double ss = 0.0;
+    for (int64_t o = 0; o < p; ++o) ss += d[o] * d[o];
+    diag->innovation_norm = sqrt(ss);
+  }
+  /* 6. S = Pf o L_o + R (:258-270) */
+  for (int64_t a = 0; a < p; ++a)
+    for (int64_t b = 0; b < p; ++b) {
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += (w[j] * Yp[(size_t)j * p + a]) * Yp[(size_t)j * p + b];
+      double dist = (double)(a > b ? a - b : b - a) / (double)p;
+      S[(size_t)a * p + b] = s * orc_lw_taper(dist, radius, fn, gc_standard) +
+                             (a == b ? var[a] : 0.0);
+    }
+  rc = orc_lu_inverse((int)p, S, Sinv);
+  if (rc) goto done;
+  /* K = (X' Y'^T) S^-1 / (k-1), then o L_g (:272-279) */
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t b = 0; b < p; ++b) {
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += Xp[(size_t)j * n + i] * Yp[(size_t)j * p + b];
+      XY[(size_t)i * p + b] = s;
+    }
+  diag->max_kalman_gain = -DBL_MAX;
+  diag->min_kalman_gain = DBL_MAX;
+  {
+    double gdim = (double)(n > p ? n : p);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t b = 0; b < p; ++b) {
+        double s = 0.0;
+        for (int64_t a = 0; a < p; ++a) s += XY[(size_t)i * p + a] * Sinv[(size_t)a * p + b];
+        double dist = (double)(i > b ? i - b : b - i) / gdim;
+        s = (s / (double)(k - 1)) * orc_lw_taper(dist, radius, fn, gc_standard);
+        K[(size_t)i * p + b] = s;
+        if (s > diag->max_kalman_gain) diag->max_kalman_gain = s;
+        if (s < diag->min_kalman_gain) diag->min_kalman_gain = s;
+      }
+  }
+  if (Kout) memcpy(Kout, K, sizeof(double) * (size_t)n * p);
+  /* cond(S) = s_max / s_min (:280-284) */
+  rc = orc_singular_values((int)p, (int)p, S, sv);
+  if (rc) goto done;
+  diag->condition_number = sv[0] / sv[p - 1];
+  /* 7-8. perturbed observations and the member updates (:286-301) */
+  for (int j = 0; j < k; ++j) {
+    for (int64_t o = 0; o < p; ++o)
+      di[o] = (yo[o] + sqrt(var[o]) * Z[(size_t)o * k + j]) - Yb[(size_t)j * p + o];
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int64_t o = 0; o < p; ++o) s += K[(size_t)i * p + o] * di[o];
+      X[(size_t)j * n + i] = mean[i] + (Xp[(size_t)j * n + i] + s);
+    }
+  }
+  /* 10. analysis spread (:304-320) */
+  orc_ensemble_mean(k, n, X, mean);
+  {
+    double ss = 0.0;
+    for (int j = 0; j < k; ++j)
+      for (int64_t i = 0; i < n; ++i) {
+        double v = X[(size_t)j * n + i] - mean[i];
+        ss += v * v;
+      }
+    diag->analysis_spread = sqrt(ss / (double)((size_t)k * n));
+  }
+done:
+  free(mean); free(Xp); free(ybar); free(Yp); free(d); free(S); free(Sinv);
+  free(XY); free(K); free(sv); free(di); free(w);
+  return rc;
+}
+
+/* ======================================================================== */
 /* ETKF                                                                       */
 /* ======================================================================== */
 

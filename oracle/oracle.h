@@ -69,6 +69,30 @@ int orc_enkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
                      const double* yo, const double* var, const double* Z,
                      double inflation, orc_enkf_diag* diag);
 
+/* ---- LWEnKF (LWEnKF.hpp:207-334, 413-668, Z injected) -------------------- */
+enum { ORC_LW_GAUSSIAN = 0, ORC_LW_EXPONENTIAL = 1, ORC_LW_CUTOFF = 2, ORC_LW_GASPARI_COHN = 3 };
+enum { ORC_LW_UNIFORM = 0, ORC_LW_ADAPTIVE = 1, ORC_LW_INVERSE_VAR = 2, ORC_LW_LIKELIHOOD = 3 };
+typedef struct {
+  double innovation_norm;
+  double background_spread;
+  double analysis_spread;
+  double max_kalman_gain;
+  double min_kalman_gain;
+  double condition_number;
+  double max_weight;
+  double min_weight;
+  double weight_variance;
+} orc_lwenkf_diag;
+
+/* the taper f(dist / radius); gc_standard selects Gaspari & Cohn (1999) for
+ * "gaspari_cohn" instead of the reference's polynomial */
+double orc_lw_taper(double dist, double radius, int fn, int gc_standard);
+/* weights[k] out; K[n][p] out when not NULL (the tapered gain) */
+int orc_lwenkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
+                       const double* yo, const double* var, const double* Z, double inflation,
+                       double radius, int fn, int gc_standard, int weighting, double* weights,
+                       double* K, orc_lwenkf_diag* diag);
+
 /* ---- ETKF (ETKF.hpp:100-178) ------------------------------------------- */
 int orc_etkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
                      const double* yo, const double* var, double inflation);

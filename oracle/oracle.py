@@ -42,6 +42,16 @@ class EnkfDiag(C.Structure):
                 ("observation_count", _i64)]
 
 
+class LwenkfDiag(C.Structure):
+    _fields_ = [("innovation_norm", _d), ("background_spread", _d), ("analysis_spread", _d),
+                ("max_kalman_gain", _d), ("min_kalman_gain", _d), ("condition_number", _d),
+                ("max_weight", _d), ("min_weight", _d), ("weight_variance", _d)]
+
+
+LW_LOC = {"gaussian": 0, "exponential": 1, "cutoff": 2, "gaspari_cohn": 3}
+LW_WEIGHT = {"uniform": 0, "adaptive": 1, "inverse_var": 2, "likelihood": 3}
+
+
 class LetkfProblem(C.Structure):
     _fields_ = [("k", C.c_int), ("nx", _i64), ("ny", _i64), ("nlev", _i64),
                 ("coord", C.c_int), ("col_a", _pd), ("col_b", _pd), ("col_c", _pd),
@@ -84,6 +94,11 @@ def lib():
         L.orc_enkf_analyse.argtypes = [C.c_int, _i64, _i64, _pd, _pd, _pd, _pd, _pd, _d,
                                        C.POINTER(EnkfDiag)]
         L.orc_etkf_analyse.argtypes = [C.c_int, _i64, _i64, _pd, _pd, _pd, _pd, _d]
+        L.orc_lw_taper.restype = _d
+        L.orc_lw_taper.argtypes = [_d, _d, C.c_int, C.c_int]
+        L.orc_lwenkf_analyse.argtypes = [C.c_int, _i64, _i64, _pd, _pd, _pd, _pd, _pd, _d, _d,
+                                         C.c_int, C.c_int, C.c_int, _pd, _pd,
+                                         C.POINTER(LwenkfDiag)]
         L.orc_letkf_standard.argtypes = [C.POINTER(LetkfProblem), _pd, _pd, _pi64, _i64, _pd,
                                          _pi64, C.c_int]
         L.orc_letkf_refcompat_snapshot.argtypes = [C.POINTER(LetkfProblem), _pd, _pd, _pd]
@@ -200,6 +215,24 @@ def enkf_analyse(X, Yb, yo, var, Z, inflation):
     _check(lib().orc_enkf_analyse(k, n, p, _p(X), _p(Yb), _p(yo), _p(var), _p(Z), inflation,
                                   C.byref(dg)), "enkf")
     return X, {f: getattr(dg, f) for f, _ in EnkfDiag._fields_}
+
+
+def lw_taper(dist, radius, fn="gaussian", gc_standard=False):
+    return lib().orc_lw_taper(float(dist), float(radius), LW_LOC[fn], int(gc_standard))
+
+
+def lwenkf_analyse(X, Yb, yo, var, Z, inflation, radius, fn="gaussian", weighting="uniform",
+                   gc_standard=False, want_K=False):
+    """Returns (X_a, weights, diag dict, K or None). Z is (p, k), Yb (k, p)."""
+    X = _f64(X).copy(); Yb = _f64(Yb); yo = _f64(yo); var = _f64(var); Z = _f64(Z)
+    k, n = X.shape; p = Yb.shape[1]
+    w = np.zeros(k)
+    K = np.zeros((n, p)) if want_K else None
+    dg = LwenkfDiag()
+    _check(lib().orc_lwenkf_analyse(k, n, p, _p(X), _p(Yb), _p(yo), _p(var), _p(Z), inflation,
+                                    radius, LW_LOC[fn], int(gc_standard), LW_WEIGHT[weighting],
+                                    _p(w), _p(K) if want_K else None, C.byref(dg)), "lwenkf")
+    return X, w, {f: getattr(dg, f) for f, _ in LwenkfDiag._fields_}, K
 
 
 def etkf_analyse(X, Yb, yo, var, inflation):

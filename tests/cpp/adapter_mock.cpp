@@ -16,14 +16,17 @@
 //   Observation iteration over points with .location, getObservationValues,
 //   getCovariance (Observation.hpp:261-482; GridObservation.hpp:197-252);
 //   ObsOperator::apply(state, obs) (ObsOperator.hpp:255-259), here the IDW4
-//   operator of IdentityObsOperator through the oracle (oracle.h).
+//   operator of IdentityObsOperator through the oracle (oracle.h);
+//   Model<Tag> forwarding to its backend with the error wrapping of
+//   Model.hpp:57-177, over metada_amd::Lorenz96Model.
 //
 // usage: adapter_mock <input file> <output prefix>
-//   input: algo form nx ny nlev k p inflation radius
+//   input: algo form nx ny nlev k p inflation radius  (or the l96 form of run_l96)
 //          k*n member values, p lines "x y z value error", p*k Z values (EnKF)
 //   output: <prefix>.analysis (k*n values, %.17g), <prefix>.mean, diagnostics
 //           on stdout, and the files saveEnsemble() writes.
 #include <cmath>
+#include <concepts>
 #include <cstdio>
 #include <limits>
 #include <fstream>
@@ -33,6 +36,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <variant>
 #include <vector>
 
@@ -72,48 +76,62 @@ struct Geometry {  // SimpleGeometry: y-major list of grid points
   auto end() const { return pts.end(); }
 };
 
-class State {  // State<Tag> over a SimpleState-like backend
+class SimpleState {  // the state backend: getData() is the raw buffer (SimpleState.hpp:69,88)
+ public:
+  explicit SimpleState(std::vector<double> v) : data_(std::move(v)) {}
+  SimpleState(const SimpleState&) = delete;
+  SimpleState& operator=(const SimpleState&) = delete;
+  SimpleState(SimpleState&&) = default;
+  SimpleState& operator=(SimpleState&&) = default;
+  void* getData() { return data_.data(); }
+  const void* getData() const { return data_.data(); }
+  size_t size() const { return data_.size(); }
+  std::vector<double> data_;
+};
+
+class State {  // State<Tag> over a SimpleState backend
  public:
   State(const Geometry* g, int nx, int ny, std::vector<double> v)
-      : g_(g), nx_(nx), ny_(ny), data_(std::move(v)) {}
-  State clone() const { return State(g_, nx_, ny_, data_); }
-  template <class T> T& getData() { return *static_cast<T*>(backendGetData()); }
-  template <class T> T* getDataPtr() { return static_cast<T*>(backendGetData()); }
-  template <class T> const T* getDataPtr() const { return static_cast<const T*>((const void*)data_.data()); }
-  size_t size() const { return data_.size(); }
+      : g_(g), nx_(nx), ny_(ny), backend_(std::move(v)) {}
+  State clone() const { return State(g_, nx_, ny_, backend_.data_); }
+  template <class T> T& getData() { return *static_cast<T*>(backend_.getData()); }
+  template <class T> T* getDataPtr() { return static_cast<T*>(backend_.getData()); }
+  template <class T> const T* getDataPtr() const { return static_cast<const T*>(backend_.getData()); }
+  size_t size() const { return backend_.size(); }
+  SimpleState& backend() { return backend_; }
+  const SimpleState& backend() const { return backend_; }
   const Geometry* geometry() const { return g_; }
-  void zero() { std::fill(data_.begin(), data_.end(), 0.0); }
+  void zero() { std::fill(backend_.data_.begin(), backend_.data_.end(), 0.0); }
   State& operator+=(const State& o) {
-    for (size_t i = 0; i < data_.size(); ++i) data_[i] += o.data_[i];
+    for (size_t i = 0; i < backend_.data_.size(); ++i) backend_.data_[i] += o.backend_.data_[i];
     return *this;
   }
   State& operator-=(const State& o) {
-    for (size_t i = 0; i < data_.size(); ++i) data_[i] -= o.data_[i];
+    for (size_t i = 0; i < backend_.data_.size(); ++i) backend_.data_[i] -= o.backend_.data_[i];
     return *this;
   }
   State& operator*=(double s) {
-    for (double& v : data_) v *= s;
+    for (double& v : backend_.data_) v *= s;
     return *this;
   }
   void saveToFile(const std::string& name) const {
     std::ofstream f(name);
     if (!f) throw std::runtime_error("cannot open " + name);
-    const size_t levels = data_.size() / ((size_t)nx_ * ny_);
+    const size_t levels = backend_.data_.size() / ((size_t)nx_ * ny_);
     for (size_t r = 0; r < (size_t)ny_ * levels; ++r) {
       for (int x = 0; x < nx_; ++x) {
         if (x) f << " ";
-        f << std::fixed << std::setprecision(6) << std::setw(12) << data_[r * nx_ + x];
+        f << std::fixed << std::setprecision(6) << std::setw(12) << backend_.data_[r * nx_ + x];
       }
       f << "\n";
     }
   }
-  const std::vector<double>& raw() const { return data_; }
+  const std::vector<double>& raw() const { return backend_.data_; }
 
  private:
-  void* backendGetData() { return data_.data(); }  // SimpleState::getData
   const Geometry* g_;
   int nx_, ny_;
-  std::vector<double> data_;
+  SimpleState backend_;
 };
 
 class Ensemble {
@@ -191,16 +209,19 @@ class ObsOperator {  // IdentityObsOperator's IDW4 (IdentityObsOperator.hpp:594-
   int nx_, ny_, nz_;
 };
 
-class ConfigValue {  // reals are float (ConfigValue.hpp:108)
+class ConfigValue {  // reals are float (ConfigValue.hpp:77-108)
  public:
   ConfigValue() = default;
+  ConfigValue(int i) : v_(i) {}
   ConfigValue(float f) : v_(f) {}
   ConfigValue(std::string s) : v_(std::move(s)) {}
+  bool isInt() const { return std::holds_alternative<int>(v_); }
+  int asInt() const { return std::get<int>(v_); }
   float asFloat() const { return std::get<float>(v_); }
   const std::string& asString() const { return std::get<std::string>(v_); }
 
  private:
-  std::variant<std::monostate, float, std::string> v_;
+  std::variant<std::monostate, int, float, std::string> v_;
 };
 
 class Config {
@@ -212,6 +233,11 @@ class Config {
       if (kv.first == key) return kv.second;
     return def;  // Config::Get swallows a missing key into the default
   }
+  bool HasKey(const std::string& key) const {  // YamlConfig.hpp:149
+    for (const auto& kv : values)
+      if (kv.first == key) return true;
+    return false;
+  }
   Config GetSubsection(const std::string& key) const {
     for (const auto& kv : subsections)
       if (kv.first == key) return kv.second;
@@ -219,8 +245,104 @@ class Config {
   }
 };
 
+// Model<Tag> (Model.hpp:57-177): forwards to the backend, wrapping errors
+template <class Backend>
+class Model {
+ public:
+  explicit Model(const Config& config) : backend_(config) {}
+  void initialize(const Config& config) {
+    if (backend_.isInitialized()) return;
+    try {
+      backend_.initialize(config);
+    } catch (const std::exception& e) {
+      throw std::runtime_error(std::string("Model initialization failed: ") + e.what());
+    }
+  }
+  bool isInitialized() const { return backend_.isInitialized(); }
+  void run(const State& initial, State& final_state) {
+    if (!backend_.isInitialized()) throw std::runtime_error("Model not initialized");
+    try {
+      backend_.run(initial.backend(), final_state.backend());
+    } catch (const std::exception& e) {
+      throw std::runtime_error(std::string("Model run failed: ") + e.what());
+    }
+  }
+  void reset() { backend_.reset(); }
+  void finalize() {
+    if (backend_.isInitialized()) backend_.finalize();
+  }
+  Backend& backend() { return backend_; }
+
+ private:
+  Backend backend_;
+};
+
+// The requirements ModelBackendImpl places on a model backend
+// (ModelConcepts.hpp:51-71), checked against metada_amd::Lorenz96Model.
+template <typename T, typename ConfigBackend, typename StateBackend>
+concept ModelBackendImpl =
+    requires(T& model, const ConfigBackend& config, const StateBackend& initialState,
+             StateBackend& finalState) {
+      { T(config) } -> std::same_as<T>;
+      { model.initialize(config) } -> std::same_as<void>;
+      { model.reset() } -> std::same_as<void>;
+      { model.finalize() } -> std::same_as<void>;
+      { model.run(initialState, finalState) } -> std::same_as<void>;
+      requires !std::is_default_constructible_v<T>;
+      requires !std::is_copy_constructible_v<T>;
+      requires !std::is_copy_assignable_v<T>;
+    };
+static_assert(ModelBackendImpl<metada_amd::Lorenz96Model, Config, SimpleState>);
+
 struct Tag {};
 }  // namespace ref
+
+// l96 <scheme> <N> <k> <num_steps> <dt> <forcing>, then k*N values: each
+// member through Model<Lorenz96Model>::run into a fresh state, and the whole
+// ensemble through runEnsemble; writes <prefix>.single and <prefix>.ensemble.
+static int run_l96(std::ifstream& in, const std::string& out) {
+  std::string scheme;
+  int N, k, steps;
+  float dt, forcing;
+  in >> scheme >> N >> k >> steps >> dt >> forcing;
+  std::vector<ref::State> members;
+  ref::Geometry geom(N, 1);
+  for (int j = 0; j < k; ++j) {
+    std::vector<double> v(N);
+    for (auto& x : v) in >> x;
+    members.emplace_back(&geom, N, 1, std::move(v));
+  }
+  if (!in) throw std::runtime_error("input file is short");
+  ref::Config cfg;
+  cfg.values = {{"scheme", ref::ConfigValue(scheme)}, {"num_steps", ref::ConfigValue(steps)},
+                {"dt", ref::ConfigValue(dt)}, {"forcing", ref::ConfigValue(forcing)}};
+  ref::Model<metada_amd::Lorenz96Model> model(cfg);
+  model.initialize(cfg);
+  std::ofstream fs(out + ".single"), fe(out + ".ensemble");
+  fs << std::setprecision(17);
+  fe << std::setprecision(17);
+  for (int j = 0; j < k; ++j) {
+    ref::State fc(&geom, N, 1, std::vector<double>(N, 0.0));
+    model.run(members[j], fc);
+    for (double v : fc.raw()) fs << v << "\n";
+  }
+  ref::Ensemble ens(std::move(members));
+  model.backend().runEnsemble(ens);
+  for (int j = 0; j < k; ++j)
+    for (double v : ens.GetMember(j).raw()) fe << v << "\n";
+  // a bad scheme surfaces through Model's error wrapping
+  ref::Config bad;
+  bad.values = {{"scheme", ref::ConfigValue(std::string("leapfrog"))}};
+  try {
+    ref::Model<metada_amd::Lorenz96Model> m2(bad);
+    std::printf("bad_scheme accepted\n");
+  } catch (const std::exception& e) {
+    std::printf("bad_scheme %s\n", e.what());
+  }
+  model.finalize();
+  std::printf("initialized_after_finalize %d\n", (int)model.isInitialized());
+  return 0;
+}
 
 int main(int argc, char** argv) {
   if (argc != 3) {
@@ -230,9 +352,11 @@ int main(int argc, char** argv) {
   try {
     std::ifstream in(argv[1]);
     std::string algo, form;
+    in >> algo;
+    if (algo == "l96") return run_l96(in, argv[2]);
     int nx, ny, nlev, k, p;
     double inflation, radius;
-    in >> algo >> form >> nx >> ny >> nlev >> k >> p >> inflation >> radius;
+    in >> form >> nx >> ny >> nlev >> k >> p >> inflation >> radius;
     const size_t n = (size_t)nx * ny * nlev;
     ref::Geometry geom(nx, ny);
     std::vector<ref::State> members;
@@ -250,7 +374,7 @@ int main(int argc, char** argv) {
       obs.pts.push_back({ref::Location(x, y, z), val, err, std::isfinite(err)});
     }
     std::vector<double> Z((size_t)p * k);
-    if (algo == "enkf")
+    if (algo == "enkf" || algo == "lwenkf")
       for (auto& z : Z) in >> z;
     if (!in) throw std::runtime_error("input file is short");
     const std::string out = argv[2];
@@ -275,6 +399,29 @@ int main(int argc, char** argv) {
       metada_amd::ETKF<ref::Tag> f(ens, obs, op, analysis, ctx);
       f.Analyse();
       f.saveEnsemble();
+    } else if (algo == "lwenkf") {
+      // form = <localization_function>:<weighting_scheme>[:standard]
+      const size_t c1 = form.find(':');
+      const size_t c2 = form.find(':', c1 + 1);
+      analysis.values.push_back({"localization_function", ref::ConfigValue(form.substr(0, c1))});
+      analysis.values.push_back(
+          {"weighting_scheme", ref::ConfigValue(form.substr(c1 + 1, c2 - c1 - 1))});
+      const bool standard_gc = c2 != std::string::npos && form.substr(c2 + 1) == "standard";
+      ref::Config root;
+      root.subsections = {{"analysis", analysis}};
+      metada_amd::LWEnKF<ref::Tag> f(ens, obs, op, root, Z, standard_gc, ctx);
+      f.Analyse();
+      f.saveEnsemble();
+      const metada_amd::LWAnalysisResults r = f.getAnalysisResults();
+      std::printf("innovation_norm %.17g\nbackground_spread %.17g\nanalysis_spread %.17g\n"
+                  "max_kalman_gain %.17g\nmin_kalman_gain %.17g\ncondition_number %.17g\n"
+                  "max_weight %.17g\nmin_weight %.17g\nweight_variance %.17g\n"
+                  "localization_function %s\nweighting_scheme %s\ninflation_method %s\n"
+                  "localization_radius %.17g\nperturbations %zu\n",
+                  r.innovation_norm, r.background_spread, r.analysis_spread, r.max_kalman_gain,
+                  r.min_kalman_gain, r.condition_number, r.max_weight, r.min_weight,
+                  r.weight_variance, r.localization_function.c_str(), r.weighting_scheme.c_str(),
+                  r.inflation_method.c_str(), r.localization_radius, ens.perturbations());
     } else {
       ref::Config root;
       root.subsections = {{"analysis", analysis}};

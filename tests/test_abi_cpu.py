@@ -110,7 +110,7 @@ def build_adapter_mock(out_dir):
     orc.build()
     libdir, orcdir = os.path.join(root, "metada_amd"), os.path.join(root, "oracle")
     exe = os.path.join(str(out_dir), "adapter_mock")
-    r = subprocess.run([gxx, "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror",
+    r = subprocess.run([gxx, "-std=c++20", "-O1", "-Wall", "-Wextra", "-Werror",
                         "-I", os.path.join(root, "include"), "-I", orcdir,
                         os.path.join(root, "tests", "cpp", "adapter_mock.cpp"), "-o", exe,
                         "-L", libdir, "-lmetada_amd", "-L", orcdir, "-loracle",
