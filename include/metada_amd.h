@@ -187,6 +187,9 @@ typedef struct { /* LWEnKF::AnalysisResults, LWEnKF.hpp:66-84 */
   int32_t localization;
   int32_t weighting;
   double inflation_factor;
+  /* device time of this call's stages, ms (no counterpart in the reference) */
+  double ms_gain_solve; /* S^-T Y' by Gauss-Jordan */
+  double ms_update;     /* the fused gain and member update (one launch) */
 } mda_lwenkf_diag;
 
 /* Local weighted EnKF: member weights w, S = L_o o (sum_i w_i y'_i y'_i^T) + R,

@@ -75,7 +75,8 @@ class LwenkfDiag(C.Structure):
                 ("min_kalman_gain", _d), ("condition_number", _d), ("localization_radius", _d),
                 ("max_weight", _d), ("min_weight", _d), ("weight_variance", _d),
                 ("ensemble_size", _i32), ("observation_count", _i64), ("inflation_method", _i32),
-                ("localization", _i32), ("weighting", _i32), ("inflation_factor", _d)]
+                ("localization", _i32), ("weighting", _i32), ("inflation_factor", _d),
+                ("ms_gain_solve", _d), ("ms_update", _d)]
 
 
 class LetkfParams(C.Structure):

@@ -554,8 +554,14 @@ class LWEnKF(_AlgorithmBase):
         d["inflation_method"] = self.inflation_method_name
         d["localization_function"] = self.localization_function
         d["weighting_scheme"] = self.weighting_scheme
-        del d["localization"], d["weighting"]
+        del d["localization"], d["weighting"], d["ms_gain_solve"], d["ms_update"]
         return d
+
+    def stage_ms(self) -> dict:
+        """Device time of the last Analyse()'s S^-T Y' solve and fused update."""
+        if self._diag is None:
+            raise RuntimeError("Analyse() has not been called")
+        return {"ms_gain_solve": self._diag.ms_gain_solve, "ms_update": self._diag.ms_update}
 
 
 class ETKF(_AlgorithmBase):

@@ -595,7 +595,9 @@ int mda_lwenkf_analyse(mda_ctx* c, const mda_lwenkf_params* prm, mda_ensemble* e
     // G^T = S^-T Y' (p x k), in place of Y'
     int64_t* piv = c->get<int64_t>("lw_piv", 1);
     double* pval = c->get<double>("lw_pval", 1);
+    MDA_HIP(hipEventRecord(c->ev[0], s));
     dense_gauss_jordan(s, p, k, St, yp, piv, pval, st);
+    MDA_HIP(hipEventRecord(c->ev[1], s));
     double* part2 = c->get<double>("sumsq_part", 1024);
     if (diag) {
       launch_dev_sumsq(s, de.X, k, n, scale, part2, 512);
@@ -610,8 +612,10 @@ int mda_lwenkf_analyse(mda_ctx* c, const mda_lwenkf_params* prm, mda_ensemble* e
     // gain and update, fused (:272-301)
     const size_t nbu = lw_update_blocks(n);
     double* mm = c->get<double>("lw_mm", 2 * nbu);
+    MDA_HIP(hipEventRecord(c->ev[2], s));
     launch_lw_update(s, de.X, mean, scale, yp, D, k, n, p, radius, prm->localization,
                      prm->gc_form, mm);
+    MDA_HIP(hipEventRecord(c->ev[3], s));
     std::vector<double> hm(2 * nbu);
     d2h(c, hm.data(), mm, sizeof(double) * 2 * nbu);
     MDA_HIP(hipStreamSynchronize(s));
@@ -639,6 +643,8 @@ int mda_lwenkf_analyse(mda_ctx* c, const mda_lwenkf_params* prm, mda_ensemble* e
       dg.weighting = prm->weighting;
       dg.inflation_factor = prm->inflation;
       dg.analysis_increment_norm = 0.0;
+      dg.ms_gain_solve = elapsed(c->ev[0], c->ev[1]);
+      dg.ms_update = elapsed(c->ev[2], c->ev[3]);
     }
     unstage_ensemble(c, ens, de);
     if (weights) std::copy(w.begin(), w.end(), weights);

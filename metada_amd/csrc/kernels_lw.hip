@@ -229,6 +229,136 @@ __global__ __launch_bounds__(256) void k_lw_update(
   if (tid == 0) mm[2 * blockIdx.x + 1] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// The same gain and update for k <= 64 with register blocking: the two
+// products are K_tile = X'_rho^T G (64 points x 32 observations) and
+// acc += K_tile D (64 points x 64 members), one after the other per chunk of
+// observations, with the taper in between.  Each thread holds a 4 x 2 tile of
+// K and a 4 x 4 tile of the accumulator, so every LDS value it reads feeds
+// two to four FMAs (the one-value-per-FMA LDS broadcast is what bounds the
+// unblocked form).  Members are padded to 64 with zeros.
+//   phase 1: thread (ty, tx) = (tid / 16, tid % 16): points ty + 16 r,
+//            observations tx + 16 t
+//   phase 2: points tx + 16 r, members ty + 16 t (stores coalesce along points)
+// ---------------------------------------------------------------------------
+constexpr int LB_TN = 64, LB_TB = 32, LB_KP = 64;
+
+// K tile entry (observation bb, point a) of the tile buffer: XOR-swizzled
+// within 16-point groups, so the phase-1 stores (16 observations at one
+// point per half-wave) and the phase-2 loads (16 points at one observation)
+// both fall in distinct banks.  80 KB of LDS in all: two workgroups per CU.
+__device__ inline int lb_ps(int bb, int a) { return bb * LB_TN + (a ^ (bb & 15)); }
+
+__global__ __launch_bounds__(256) void k_lw_update_blk(
+    double* __restrict__ X, const double* __restrict__ mean, double scale,
+    const double* __restrict__ Gt, const double* __restrict__ D, int k, int64_t n, int64_t p,
+    double radius, int fn, int gc_form, double* __restrict__ mm) {
+  extern __shared__ double sm[];
+  double* Qs = sm;                          // [LB_KP][LB_TN]  X'_rho, member-major
+  double* Gs = Qs + LB_KP * LB_TN;          // [LB_KP][LB_TB]  G chunk, member-major
+  double* Ds = Gs + LB_KP * LB_TB;          // [LB_TB][LB_KP]  D chunk
+  double* Ps = Ds + LB_TB * LB_KP;          // [LB_TB][LB_TN]  tapered K tile (lb_ps)
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const int64_t s0 = (int64_t)blockIdx.x * LB_TN;
+  for (int e = tid; e < LB_KP * LB_TN; e += 256) {
+    const int i = e / LB_TN, a = e % LB_TN;
+    const int64_t sa = s0 + a;
+    Qs[e] = (i < k && sa < n) ? (X[(size_t)i * n + sa] - mean[sa]) * scale : 0.0;
+  }
+  const double kden = (double)(k - 1);
+  const double gdim = (double)(n > p ? n : p);
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[r][t] = 0.0;
+  double kmax = -INFINITY, kmin = INFINITY;
+  for (int64_t b0 = 0; b0 < p; b0 += LB_TB) {
+    const int nb = (int)(p - b0 < LB_TB ? p - b0 : LB_TB);
+    __syncthreads();  // the previous chunk is consumed (and Qs is complete)
+    for (int e = tid; e < LB_TB * LB_KP; e += 256) {
+      const int bb = e / LB_KP, i = e % LB_KP;
+      const bool in = bb < nb && i < k;
+      const size_t at = (size_t)(b0 + bb) * k + i;
+      Gs[i * LB_TB + bb] = in ? Gt[at] : 0.0;
+      Ds[bb * LB_KP + i] = in ? D[at] : 0.0;
+    }
+    __syncthreads();
+    // phase 1: K for points ty + 16 r and observations tx + 16 t
+    double kt[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) kt[r][0] = kt[r][1] = 0.0;
+    for (int i = 0; i < k; ++i) {
+      double q[4], g[2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[r] = Qs[i * LB_TN + ty + 16 * r];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) g[t] = Gs[i * LB_TB + tx + 16 * t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) kt[r][t] = fma(q[r], g[t], kt[r][t]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int a = ty + 16 * r, bb = tx + 16 * t;
+        const int64_t sa = s0 + a, ob = b0 + bb;
+        double kk = 0.0;
+        if (sa < n && bb < nb) {
+          const double dist = (double)(sa > ob ? sa - ob : ob - sa) / gdim;
+          kk = (kt[r][t] / kden) * lw_taper(dist, radius, fn, gc_form);
+          kmax = fmax(kmax, kk);
+          kmin = fmin(kmin, kk);
+        }
+        Ps[lb_ps(bb, a)] = kk;
+      }
+    __syncthreads();
+    // phase 2: acc += K D for points tx + 16 r and members ty + 16 t
+    for (int bb = 0; bb < nb; ++bb) {
+      double pk[4], dv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pk[r] = Ps[lb_ps(bb, tx + 16 * r)];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) dv[t] = Ds[bb * LB_KP + ty + 16 * t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[r][t] = fma(pk[r], dv[t], acc[r][t]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = tx + 16 * r;
+    const int64_t sa = s0 + a;
+    if (sa >= n) continue;
+    const double md = mean[sa];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int m = ty + 16 * t;
+      if (m < k) X[(size_t)m * n + sa] = md + (Qs[m * LB_TN + a] + acc[r][t]);
+    }
+  }
+  double* red = Ps;  // the tile buffer is free after the last chunk
+  __syncthreads();
+  red[tid] = kmax;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmax(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) mm[2 * blockIdx.x] = red[0];
+  __syncthreads();
+  red[tid] = kmin;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmin(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) mm[2 * blockIdx.x + 1] = red[0];
+}
+
 // any +inf variance (an invalid observation) -> *flag = 1
 __global__ void k_lw_flag_inf(const double* __restrict__ var, int64_t p, int* __restrict__ flag) {
   bool bad = false;
@@ -269,6 +399,7 @@ void launch_lw_innovations(hipStream_t s, const double* yo, const double* ybar,
   MDA_LAUNCH_CHECK();
 }
 
+static_assert(LB_TN == LW_TN, "both update forms cover the same points per block");
 size_t lw_update_blocks(int64_t n) { return (size_t)((n + LW_TN - 1) / LW_TN); }
 
 void launch_lw_update(hipStream_t s, double* X, const double* mean, double scale,
@@ -276,6 +407,14 @@ void launch_lw_update(hipStream_t s, double* X, const double* mean, double scale
                       double radius, int fn, int gc_form, double* mm) {
   MDA_REQUIRE(k >= 2 && k <= 128, "LWEnKF supports 2 <= k <= 128");
   const unsigned nb = (unsigned)lw_update_blocks(n);
+  if (k <= LB_KP) {
+    const size_t shb = sizeof(double) * ((size_t)LB_KP * LB_TN + (size_t)LB_KP * LB_TB +
+                                         (size_t)LB_TB * LB_KP + (size_t)LB_TB * LB_TN);
+    hipLaunchKernelGGL(k_lw_update_blk, dim3(nb), dim3(256), shb, s, X, mean, scale, Gt, D, k, n,
+                       p, radius, fn, gc_form, mm);
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   const size_t shm = sizeof(double) * ((size_t)k * LW_TN + (size_t)k * LW_TB +
                                        (size_t)LW_TB * k + (size_t)LW_TB * LW_TN);
   if (k <= 32) {

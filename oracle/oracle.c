@@ -553,7 +553,7 @@ double orc_lw_taper(double dist, double radius, int fn, int gc_standard) {
 int orc_lwenkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
                        const double* yo, const double* var, const double* Z, double inflation,
                        double radius, int fn, int gc_standard, int weighting, double* weights,
-                       double* Kout, orc_lwenkf_diag* diag) {
+                       double* Kout, int want_cond, orc_lwenkf_diag* diag) {
   if (k < 2 || n <= 0 || p <= 0 || !(radius > 0.0)) return ORC_E_ARG;
   size_t nk = (size_t)n * k, pk = (size_t)p * k, pp = (size_t)p * p;
   double* mean = (double*)malloc(sizeof(double) * n);
@@ -666,9 +666,13 @@ int orc_lwenkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
   }
   if (Kout) memcpy(Kout, K, sizeof(double) * (size_t)n * p);
   /* cond(S) = s_max / s_min (:280-284) */
-  rc = orc_singular_values((int)p, (int)p, S, sv);
-  if (rc) goto done;
-  diag->condition_number = sv[0] / sv[p - 1];
+  if (want_cond) {
+    rc = orc_singular_values((int)p, (int)p, S, sv);
+    if (rc) goto done;
+    diag->condition_number = sv[0] / sv[p - 1];
+  } else {
+    diag->condition_number = NAN;
+  }
   /* 7-8. perturbed observations and the member updates (:286-301) */
   for (int j = 0; j < k; ++j) {
     for (int64_t o = 0; o < p; ++o)

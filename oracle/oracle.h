@@ -87,11 +87,12 @@ typedef struct {
 /* the taper f(dist / radius); gc_standard selects Gaspari & Cohn (1999) for
  * "gaspari_cohn" instead of the reference's polynomial */
 double orc_lw_taper(double dist, double radius, int fn, int gc_standard);
-/* weights[k] out; K[n][p] out when not NULL (the tapered gain) */
+/* weights[k] out; K[n][p] out when not NULL (the tapered gain); cond(S) is
+ * computed when want_cond (NaN otherwise) */
 int orc_lwenkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,
                        const double* yo, const double* var, const double* Z, double inflation,
                        double radius, int fn, int gc_standard, int weighting, double* weights,
-                       double* K, orc_lwenkf_diag* diag);
+                       double* K, int want_cond, orc_lwenkf_diag* diag);
 
 /* ---- ETKF (ETKF.hpp:100-178) ------------------------------------------- */
 int orc_etkf_analyse(int k, int64_t n, int64_t p, double* X, const double* Yb,

@@ -97,7 +97,7 @@ def lib():
         L.orc_lw_taper.restype = _d
         L.orc_lw_taper.argtypes = [_d, _d, C.c_int, C.c_int]
         L.orc_lwenkf_analyse.argtypes = [C.c_int, _i64, _i64, _pd, _pd, _pd, _pd, _pd, _d, _d,
-                                         C.c_int, C.c_int, C.c_int, _pd, _pd,
+                                         C.c_int, C.c_int, C.c_int, _pd, _pd, C.c_int,
                                          C.POINTER(LwenkfDiag)]
         L.orc_letkf_standard.argtypes = [C.POINTER(LetkfProblem), _pd, _pd, _pi64, _i64, _pd,
                                          _pi64, C.c_int]
@@ -222,7 +222,7 @@ def lw_taper(dist, radius, fn="gaussian", gc_standard=False):
 
 
 def lwenkf_analyse(X, Yb, yo, var, Z, inflation, radius, fn="gaussian", weighting="uniform",
-                   gc_standard=False, want_K=False):
+                   gc_standard=False, want_K=False, want_cond=True):
     """Returns (X_a, weights, diag dict, K or None). Z is (p, k), Yb (k, p)."""
     X = _f64(X).copy(); Yb = _f64(Yb); yo = _f64(yo); var = _f64(var); Z = _f64(Z)
     k, n = X.shape; p = Yb.shape[1]
@@ -231,7 +231,8 @@ def lwenkf_analyse(X, Yb, yo, var, Z, inflation, radius, fn="gaussian", weightin
     dg = LwenkfDiag()
     _check(lib().orc_lwenkf_analyse(k, n, p, _p(X), _p(Yb), _p(yo), _p(var), _p(Z), inflation,
                                     radius, LW_LOC[fn], int(gc_standard), LW_WEIGHT[weighting],
-                                    _p(w), _p(K) if want_K else None, C.byref(dg)), "lwenkf")
+                                    _p(w), _p(K) if want_K else None, int(want_cond),
+                                    C.byref(dg)), "lwenkf")
     return X, w, {f: getattr(dg, f) for f, _ in LwenkfDiag._fields_}, K
 
 
