@@ -52,6 +52,9 @@ def parse():
                     help="LWEnKF form instead of the headline run: a 2048 x 2048 state "
                          "(n = 4,194,304), k = 64, p = 512 IDW4 observations, adaptive weights, "
                          "gaussian index-distance taper of radius 0.05, rho = 1.05")
+    ap.add_argument("--transform-precision", default="fp64", choices=["fp64", "bf16x3"],
+                    help="precision of X_a = xbar + X'T on the timed path (the split-precision "
+                         "form is also measured beside it, with its errors, in the full run)")
     ap.add_argument("--value-path-only", action="store_true",
                     help="run only the --precision path (for profiler runs): no oracle, "
                          "no FP64 reference, no error report")
@@ -191,9 +194,11 @@ def main():
     cfg = mda.Config({"inflation": a.rho, "localization_radius": a.radius})
     ens = mda.Ensemble(W["X"], a.nlon, a.nlat, nlev)
 
-    def make(prec_code, eig):
+    tp_value = {"fp64": L.PREC_FP64, "bf16x3": L.PREC_BF16X3}[a.transform_precision]
+
+    def make(prec_code, eig, tp=L.PREC_FP64):
         return mda.LETKF(ens, W["obs"], W["H"], cfg, geometry=W["geom"], ctx=ctx,
-                         precision=prec_code, eig_method=eig)
+                         precision=prec_code, eig_method=eig, transform_precision=tp)
 
     def run(name, f, warmup, steps, check_parity=False):
         times, diags = [], []
@@ -238,7 +243,7 @@ def main():
 
     if a.value_path_only:
         code = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}[a.precision]
-        tv, dv = run(a.precision, make(code, L.EIG_AUTO), a.warmup, a.steps)
+        tv, dv = run(a.precision, make(code, L.EIG_AUTO, tp_value), a.warmup, a.steps)
         sv = summary(tv, dv)
         print(json.dumps({"metric": "profiler run", "precision": a.precision,
                           "ms_median": sv["ms_median"], "stages_ms": sv["stages_ms"]}), flush=True)
@@ -271,14 +276,21 @@ def main():
     mixed = {}
     for pname in ("fp32", "bf16x3"):
         code = {"fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}[pname]
-        fm = make(code, L.EIG_AUTO)
         is_value = pname == a.precision
+        fm = make(code, L.EIG_AUTO, tp_value if is_value else L.PREC_FP64)
         tm, dm = run(pname, fm, a.warmup if is_value else 1,
                      a.steps if is_value else max(1, min(a.steps, 3)))
         mixed[pname] = summary(tm, dm)
         mixed[pname]["error_vs_fp64"] = errors(W["X"])
         log(f"{pname}: errors vs FP64 {json.dumps(mixed[pname]['error_vs_fp64'])}")
     res.update(mixed)
+    # the precision form with X'W split-precision too (BF16x3 Gram and transform)
+    if a.transform_precision == "fp64" and k % 16 == 0 and k <= 64:
+        fx = make(L.PREC_BF16X3, L.EIG_AUTO, L.PREC_BF16X3)
+        tx, dx = run("bf16x3_xw", fx, 1, max(1, min(a.steps, 3)))
+        res["bf16x3_xw"] = summary(tx, dx)
+        res["bf16x3_xw"]["error_vs_fp64"] = errors(W["X"])
+        log(f"bf16x3_xw: errors vs FP64 {json.dumps(res['bf16x3_xw']['error_vs_fp64'])}")
     if a.precision == "fp64":
         f64b = make(L.PREC_FP64, L.EIG_AUTO)
         tv, dv = run("fp64_value", f64b, a.warmup, a.steps)
@@ -327,7 +339,8 @@ def main():
         "eig_vectors": ("k_trd_vectors", "mfma", e3 / max(stages["ms_eig_vectors"], 1e-9) / 1e9,
                         PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_vectors"],
                         "FP64 VALU issue and latency at one wave per SIMD"),
-        "transform": ("k_letkf_transform_mfma" if k % 16 == 0 and k <= 64 else
+        "transform": (("k_letkf_transform_bf16x3" if tp_value == L.PREC_BF16X3 else
+                       "k_letkf_transform_mfma") if k % 16 == 0 and k <= 64 else
                       "k_letkf_transform_reg" if k in (8, 20, 40) else "k_letkf_transform", "hbm",
                       xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
                       PEAK["hbm_GBs"], "GB/s", stages["ms_transform"],

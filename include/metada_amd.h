@@ -238,7 +238,7 @@ typedef struct {
   int32_t inflation_form;    /* A=(k-1)I/rho+C | Pa = rho (C+(k-1)I)^-1 (LETKF.hpp:219) */
   int32_t sqrt_form;         /* symmetric A^-1/2 | chol_L(Pa) (LETKF.hpp:223) */
   int32_t combine;           /* X'(wa 1^T + Wa) | x'_i (wa_i + sum_j Wa_ij) (LETKF.hpp:235-238) */
-  int32_t precision;         /* contractions in FP64 | FP32 | BF16x3 (eig always FP64) */
+  int32_t precision;         /* Y'^T R^-1 Y' in FP64 | FP32 | BF16x3 (eig always FP64) */
   const double* rho_column;  /* optional per-column rho [nx*ny] (learned inflation) */
   int32_t rho_on_device;
   int32_t eig_method;        /* symmetric square root: MDA_EIG_AUTO (Householder
@@ -246,6 +246,9 @@ typedef struct {
                                 factorisation eigenvectors; columns failing the
                                 residual/orthogonality checks are re-solved by
                                 Jacobi) | MDA_EIG_JACOBI (parallel-order Jacobi) */
+  int32_t transform_precision; /* X_a = xbar + X' T in FP64 (0, the default) | MDA_PREC_BF16X3
+                                  (split-precision bf16 products, FP32 accumulation; k a
+                                  multiple of 16, at most 64) */
 } mda_letkf_params;
 
 enum { MDA_EIG_AUTO = 0, MDA_EIG_JACOBI = 1 };

@@ -590,7 +590,8 @@ class LETKF(_AlgorithmBase):
 
     def __init__(self, ensemble, obs, obs_op, config: Config, geometry: ColumnGeometry = None,
                  ctx=None, form: str = "standard", precision: int = L.PREC_FP64,
-                 rho_column=None, eig_method: int = L.EIG_AUTO):
+                 rho_column=None, eig_method: int = L.EIG_AUTO,
+                 transform_precision: int = L.PREC_FP64):
         super().__init__(ensemble, obs, obs_op, ctx)
         self.inflation = config.Get("inflation").asFloat()
         self.localization_radius = config.Get("localization_radius").asFloat()
@@ -608,6 +609,7 @@ class LETKF(_AlgorithmBase):
             raise ValueError(f"unknown LETKF form {form!r}")
         self.params.precision = precision
         self.params.eig_method = eig_method
+        self.params.transform_precision = transform_precision
         self._rho_keep = None
         if rho_column is not None:
             rp, rdev, self._rho_keep = _buf(rho_column)

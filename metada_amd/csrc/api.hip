@@ -732,6 +732,9 @@ static void check_letkf_params(const mda_letkf_params* prm, const mda_geometry* 
   MDA_REQUIRE(prm->precision == MDA_PREC_FP64 || prm->precision == MDA_PREC_FP32 ||
                   prm->precision == MDA_PREC_BF16X3,
               "precision must be FP64, FP32 or BF16X3");
+  MDA_REQUIRE(prm->transform_precision == MDA_PREC_FP64 ||
+                  prm->transform_precision == MDA_PREC_BF16X3,
+              "transform_precision must be FP64 or BF16X3");
   MDA_REQUIRE(geom->coord >= 0 && geom->coord <= 2, "unknown coordinate system");
 }
 
@@ -753,6 +756,8 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   P.n = P.de.n; P.nx = ens->nx; P.ny = ens->ny; P.nlev = ens->nlev;
   const int64_t ncol = P.ncol = P.nx * P.ny;
   MDA_REQUIRE(k <= 128, "LETKF supports k <= 128 in this build");
+  MDA_REQUIRE(prm->transform_precision != MDA_PREC_BF16X3 || (k % 16 == 0 && k <= 64),
+              "a BF16x3 transform needs k a multiple of 16, at most 64");
   const bool gd = geom->on_device != 0;
   const double* ca = to_device(c, pre + "geom_a", geom->col_a, ncol, gd);
   const double* cb = to_device(c, pre + "geom_b", geom->col_b, ncol, gd);
@@ -1019,7 +1024,8 @@ static void enqueue_columns(hipStream_t s, letkf_plan& P, std::pair<int, int> cf
     for (int i = 0; i < 3; ++i) sa.ev_stage[i] = m ? m->e[2 + i] : nullptr;
     launch_letkf_solve(s, sa);
     if (m) MDA_HIP(hipEventRecord(m->e[5], s));
-    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, P.sa.status, c0, nc);
+    launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, P.sa.status, c0, nc,
+                           P.prm->transform_precision);
     if (m) MDA_HIP(hipEventRecord(m->e[6], s));
   }
 }

@@ -1855,6 +1855,146 @@ void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
   }
 }
 
+// 4'''. X_a = xbar + X' T with split-precision products, the transform of
+// the BF16x3 form: X' = X - xbar in FP64 (xbar over the members per level),
+// X' = hi + lo and T = hi + lo in bf16 (round to nearest even; lo from the
+// FP64 residual), X' T ~ hi hi + hi lo + lo hi on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation (A[l%16][8(l/16)+e], B[8(l/16)+e][l%16],
+// D[4(l/16)+r][l%16]), xbar added back in FP64.  Block shape as in the FP64
+// form: XM_C columns per block, one wavefront each, 16-level chunks staged
+// through LDS.  Members are padded to a multiple of 32 with zeros.
+__device__ inline void split_bf16(const double (&v)[8], bf8& hi, bf8& lo) {
+  u32x4 ph, pl;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const fl2 f = {(float)v[2 * q], (float)v[2 * q + 1]};
+    const bf2 h = __builtin_convertvector(f, bf2);
+    const fl2 hf = __builtin_convertvector(h, fl2);
+    const fl2 r = {(float)(v[2 * q] - (double)hf[0]), (float)(v[2 * q + 1] - (double)hf[1])};
+    const bf2 lw = __builtin_convertvector(r, bf2);
+    ph[q] = __builtin_bit_cast(uint32_t, h);
+    pl[q] = __builtin_bit_cast(uint32_t, lw);
+  }
+  hi = __builtin_bit_cast(bf8, ph);
+  lo = __builtin_bit_cast(bf8, pl);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_letkf_transform_bf16x3(double* __restrict__ X, int64_t ncol, int64_t nlev,
+                              const double* __restrict__ T, int64_t col0, int64_t nc,
+                              int* __restrict__ status) {
+  static_assert(K % 16 == 0 && K <= 64, "K must be a multiple of 16, at most 64");
+  if (*status & MDA_ST_SKIP_UPDATE) return;
+  constexpr int KS = (K + 31) / 32, NTJ = K / 16;
+  __shared__ double xs[XM_C * K * XM_L];  // [column][member or j][level]
+  __shared__ double xb[XM_C][XM_L];       // xbar of the chunk's levels, per column
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int64_t n = ncol * nlev;
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t g0 = vb * XM_C;
+  if (g0 >= nc) return;
+  const int ncl = (int)(nc - g0 < XM_C ? nc - g0 : XM_C);
+  const bool own = wv < ncl;
+  // T of column g0 + wv, split: B[8 kq + e][i16] of k-step s4, tile tj =
+  // T[32 s4 + 8 kq + e][16 tj + i16]
+  bf8 th[KS][NTJ], tl[KS][NTJ];
+  {
+    const double* Tg = T + (size_t)(g0 + (own ? wv : 0)) * K * K;
+#pragma unroll
+    for (int s4 = 0; s4 < KS; ++s4)
+#pragma unroll
+      for (int tj = 0; tj < NTJ; ++tj) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int m = 32 * s4 + 8 * kq + e;
+          v[e] = (own && m < K) ? Tg[(size_t)m * K + 16 * tj + i16] : 0.0;
+        }
+        split_bf16(v, th[s4][tj], tl[s4][tj]);
+      }
+  }
+  constexpr int NV = XM_C * XM_L * K / 256;
+  double pv[NV];
+  auto load_chunk = [&](int64_t l0, int nll) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 256 * q;
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
+      double v = 0.0;
+      if (c < ncl && l < nll) v = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c];
+      pv[q] = v;
+    }
+  };
+  load_chunk(0, (int)(nlev < XM_L ? nlev : XM_L));
+  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
+    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
+    __syncthreads();  // the previous chunk's output has left xs
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 256 * q;
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
+      xs[(c * K + m) * XM_L + l] = pv[q];
+    }
+    __syncthreads();
+    if (l0 + XM_L < nlev)
+      load_chunk(l0 + XM_L, (int)(nlev - l0 - XM_L < XM_L ? nlev - l0 - XM_L : XM_L));
+    // xbar of level i16: lane (i16, kq) sums members kq K/4 .. (kq + 1) K/4 - 1,
+    // then the four partial sums are combined
+    const double* xc = xs + (size_t)wv * K * XM_L;
+    double part = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < K / 4; ++mm) part += xc[(kq * (K / 4) + mm) * XM_L + i16];
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    const double xbar = part * (1.0 / (double)K);
+    if (kq == 0) xb[wv][i16] = xbar;
+    f4 acc[NTJ];
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj) acc[tj] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s4 = 0; s4 < KS; ++s4) {
+      double v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int m = 32 * s4 + 8 * kq + e;
+        v[e] = m < K ? xc[m * XM_L + i16] - xbar : 0.0;
+      }
+      bf8 ah, al;
+      split_bf16(v, ah, al);
+#pragma unroll
+      for (int tj = 0; tj < NTJ; ++tj) {
+        acc[tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, th[s4][tj], acc[tj], 0, 0, 0);
+        acc[tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, tl[s4][tj], acc[tj], 0, 0, 0);
+        acc[tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, th[s4][tj], acc[tj], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // every wavefront has read its X (and xb is complete)
+    double out[NTJ][4];
+    bool fin = true;
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        out[tj][r] = xb[wv][4 * kq + r] + (double)acc[tj][r];
+        fin = fin && isfinite(out[tj][r]);
+      }
+    if (__ballot(!fin) != 0ull && lane == 0) atomicOr(status, MDA_ST_NONFINITE_STATE);
+    // D[4 kq + r][i16] of tile tj = X_a[level 4 kq + r][j = 16 tj + i16] -> xs[c][j][l]
+#pragma unroll
+    for (int tj = 0; tj < NTJ; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xs[(wv * K + 16 * tj + i16) * XM_L + 4 * kq + r] = out[tj][r];
+    __syncthreads();
+    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
+      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, j = idx / (XM_C * XM_L);
+      if (c < ncl && l < nll) X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = xs[(c * K + j) * XM_L + l];
+    }
+  }
+}
+
 bool letkf_transform_reg_supported(int k) {
   switch (k) {
     case 8: case 16: case 20: case 32: case 40: case 48: case 64: return true;
@@ -1863,12 +2003,25 @@ bool letkf_transform_reg_supported(int k) {
 }
 
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T, int* status, int64_t col0, int64_t nc) {
+                            const double* T, int* status, int64_t col0, int64_t nc,
+                            int precision) {
   MDA_REQUIRE(status != nullptr, "transform needs the analysis status word");
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
   if (nc < 0) nc = ncol;
   MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
   if (nc == 0) return;
+  if (precision == MDA_PREC_BF16X3 && k % 16 == 0 && k <= 64) {
+    MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
+    const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
+    switch (k) {
+      case 16: hipLaunchKernelGGL(k_letkf_transform_bf16x3<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      case 32: hipLaunchKernelGGL(k_letkf_transform_bf16x3<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      case 48: hipLaunchKernelGGL(k_letkf_transform_bf16x3<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      default: hipLaunchKernelGGL(k_letkf_transform_bf16x3<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+    }
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   if (k % 16 == 0 && k <= 64) {
     MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);

@@ -341,9 +341,12 @@ void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, doub
                       int* flag);
 // X_a = xbar + X' T for the columns [col0, col0 + nc) of X (ncol columns in
 // all; nc < 0: every column); T holds the nc columns' transforms.  Skipped
-// when the status word holds a MDA_ST_SKIP_UPDATE bit.
+// when the status word holds a MDA_ST_SKIP_UPDATE bit.  MDA_PREC_BF16X3
+// runs X' T as split-precision bf16 products with FP32 accumulation (k a
+// multiple of 16, at most 64); otherwise FP64.
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
-                            const double* T, int* status, int64_t col0 = 0, int64_t nc = -1);
+                            const double* T, int* status, int64_t col0 = 0, int64_t nc = -1,
+                            int precision = MDA_PREC_FP64);
 void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,
                        int64_t* out_empty);
 

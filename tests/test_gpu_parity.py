@@ -344,6 +344,56 @@ def test_letkf_mixed_precision_contractions(mda, ctx, orc, prec, bound, k):
     assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 10 * bound * inc
 
 
+@pytest.mark.parametrize("gram", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("k", [64, 48, 32])
+def test_letkf_bf16x3_transform(mda, ctx, orc, gram, k):
+    """X_a = xbar + X' T with split-precision bf16 products and FP32
+    accumulation (transform_precision = BF16X3), alone or with the BF16x3
+    Gram: within 1e-4 relative RMS of the oracle's FP64 analysis, the
+    increment reproduced, and the mean update (which the FP64 xbar carries)
+    no worse than the perturbations."""
+    case = latlon_case(k=k, p=6000, seed=90 + k, nmodes=96)
+    L = mda._lib
+    ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                          coord=L.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(L.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+    f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                   "localization_radius": 1800.0}),
+                  geometry=geom, ctx=ctx,
+                  precision=L.PREC_BF16X3 if gram == "bf16x3" else L.PREC_FP64,
+                  transform_precision=L.PREC_BF16X3)
+    f.Analyse()
+    prob = orc.Problem(k, case["nlon"], case["nlat"], case["nlev"], orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    Xm = ens.data
+    rms = np.sqrt(np.mean((Xm - Xo) ** 2)) / np.sqrt(np.mean(Xo ** 2))
+    assert rms < 1e-4, rms
+    inc = np.sqrt(np.mean((Xo - case["X"]) ** 2))
+    assert np.sqrt(np.mean((Xm - Xo) ** 2)) < 1e-3 * inc
+    em = np.sqrt(np.mean((Xm.mean(0) - Xo.mean(0)) ** 2))
+    ep = np.sqrt(np.mean(((Xm - Xm.mean(0)) - (Xo - Xo.mean(0))) ** 2))
+    assert em <= 2 * ep + 1e-14
+
+
+def test_letkf_bf16x3_transform_refuses_unsupported_k(mda, ctx):
+    case = latlon_case(k=40, p=500, seed=3)
+    L = mda._lib
+    ens = mda.Ensemble(case["X"].copy(), case["nlon"], case["nlat"], case["nlev"])
+    obs = mda.Observation(case["yo"], case["var"], case["lat"], case["lon"],
+                          coord=L.COORD_GEOGRAPHIC)
+    geom = mda.ColumnGeometry(L.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
+    f = mda.LETKF(ens, obs, case["H"], mda.Config({"inflation": 1.05,
+                                                   "localization_radius": 1800.0}),
+                  geometry=geom, ctx=ctx, transform_precision=L.PREC_BF16X3)
+    with pytest.raises(L.MdaError) as e:
+        f.Analyse()
+    assert e.value.code == L.MDA_E_ARG
+    np.testing.assert_array_equal(ens.data, case["X"])
+
+
 @pytest.mark.parametrize("prec,bound", [("fp64", TOL), ("bf16x3", 1e-4)])
 def test_letkf_supplied_hx_matches_stencil(mda, ctx, orc, prec, bound):
     """H(x) handed over member-major (read in input order, rows moved into
