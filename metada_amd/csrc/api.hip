@@ -776,7 +776,8 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   bg.coord = geom->coord;
   if (geom->coord == MDA_COORD_GEOGRAPHIC) {
     double ddeg = prm->radius / 6371.0 * 180.0 / 3.14159265358979323846;
-    double step = std::min(10.0, std::max(0.05, ddeg / 2.0));
+    // at least 1/16 degree: 2880 x 5760 buckets stay under the 2^24 key limit
+    double step = std::min(10.0, std::max(0.0625, ddeg / 2.0));
     // bucket sizes divide the latitude and longitude ranges exactly
     bg.nrow = (int)std::ceil(180.0 / step);
     bg.ncolb = (int)std::ceil(360.0 / step);

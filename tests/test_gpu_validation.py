@@ -101,13 +101,13 @@ def _boundary_lon(orc, lat0, lon0, r):
     return x
 
 
-@pytest.mark.parametrize("radius", [1500.0, 37.0])
+@pytest.mark.parametrize("radius", [1500.0, 37.0, 5.0])
 def test_geographic_boundary_local_sets_match_oracle(mda, ctx, orc, radius):
     """Observations at the radius and one or two ulps either side, along a
     meridian and along a parallel of several columns: every column's local
     count equals the oracle's exactly, in the FP64 and the reduced-precision
     Gram and in the hard-cutoff (reference) form, whose analysis then matches
-    the oracle."""
+    the oracle.  5 km is below the bucket grid's 1/16-degree floor."""
     nlon, nlat, nlev, k = 24, 12, 2, 8
     lonc = (np.arange(nlon) + 0.5) * 360 / nlon
     latc = -90 + (np.arange(nlat) + 0.5) * 180 / nlat
