@@ -47,17 +47,6 @@ __device__ inline void wait_vmcnt() {
   lds_reload_point();
 }
 
-// Pulls `bytes` at `src` into the L2 without waiting or holding registers:
-// one dword of every 128-byte line is loaded by global_load_lds into an LDS
-// sink that nothing reads.  The lines are then L2 hits when the data is used.
-__device__ inline void l2_prefetch(const void* src, int64_t bytes, unsigned* sink) {
-  const char* p = (const char*)src;
-  const int64_t lines = (bytes + 127) / 128;
-  for (int64_t i = threadIdx.x & 63; i < lines; i += 64)
-    __builtin_amdgcn_global_load_lds((const void*)(p + i * 128),
-                                     (__attribute__((address_space(3))) void*)sink, 4, 0, 0);
-}
-
 // One DPP move of a double (both halves); lanes whose source is out of range
 // or whose row is masked off read 0.
 template <int CTRL, int ROW_MASK = 0xf>
@@ -405,12 +394,9 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
   __shared__ double Z[ZS];
   __shared__ double sd[64], se[64], sl[64], sy[64], sb[64], swt[64];
   __shared__ double dots[64][BAND];
-  __shared__ unsigned sink[64];
   const int lane = threadIdx.x;
   for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
     if (nloc[g] == 0 || flag[g]) continue;  // empty columns / Jacobi fallback columns
-    // this column's reflector rows into the L2 while its eigenvectors are formed
-    l2_prefetch(Ug + (size_t)g * n * n, (int64_t)sizeof(double) * n * n, sink);
     const double r = rho_col ? rho_col[g] : rho;
     double* Tg = T + (size_t)g * n * n;
     const double* dg = de + (size_t)g * 2 * n;
