@@ -196,9 +196,10 @@ def main():
 
     tp_value = {"fp64": L.PREC_FP64, "bf16x3": L.PREC_BF16X3}[a.transform_precision]
 
-    def make(prec_code, eig, tp=L.PREC_FP64):
+    def make(prec_code, eig, tp=L.PREC_FP64, keep=False):
         return mda.LETKF(ens, W["obs"], W["H"], cfg, geometry=W["geom"], ctx=ctx,
-                         precision=prec_code, eig_method=eig, transform_precision=tp)
+                         precision=prec_code, eig_method=eig, transform_precision=tp,
+                         keep_transforms=keep)
 
     def run(name, f, warmup, steps, check_parity=False):
         times, diags = [], []
@@ -249,11 +250,21 @@ def main():
                           "ms_median": sv["ms_median"], "stages_ms": sv["stages_ms"]}), flush=True)
         ctx.close()
         return 0
-    # FP64, tridiagonal eigensolver (fastest FP64 path): reference output
+    # FP64 with every column's T kept (eigenvectors and transform unfused):
+    # T of the sampled columns against the oracle
+    fk = make(L.PREC_FP64, L.EIG_AUTO, keep=True)
+    tk, dk = run("fp64_kept", fk, 0, 1, check_parity=True)
+    Xk = W["X"].clone()
+    # FP64, tridiagonal eigensolver with the fused transform (fastest FP64
+    # path): reference output
     f64 = make(L.PREC_FP64, L.EIG_AUTO)
-    t64, d64 = run("fp64", f64, 1, max(1, min(a.steps, 5)), check_parity=True)
+    t64, d64 = run("fp64", f64, 1, max(1, min(a.steps, 5)))
     Xd = W["X"].clone()
     res = {"fp64": summary(t64, d64)}
+    res["fp64_kept"] = summary(tk, dk)
+    res["fp64"]["max_abs_diff_vs_kept"] = float((Xd - Xk).abs().max())
+    log(f"fp64: fused vs kept-T path max |diff| = {res['fp64']['max_abs_diff_vs_kept']:.3e}")
+    del Xk
     # FP64 plain: FP64 contractions and the batched Jacobi eigensolver
     if not a.skip_plain:
         fpl = make(L.PREC_FP64, L.EIG_JACOBI)

@@ -249,6 +249,11 @@ typedef struct {
   int32_t transform_precision; /* X_a = xbar + X' T in FP64 (0, the default) | MDA_PREC_BF16X3
                                   (split-precision bf16 products, FP32 accumulation; k a
                                   multiple of 16, at most 64) */
+  int32_t keep_transforms;     /* 1: keep every column's T for mda_letkf_last_transforms.
+                                  0 (the default): at k in {16, 32, 48, 64} with the
+                                  tridiagonal eigensolver, the matrix combination and an
+                                  FP64 transform, the eigenvector kernel applies
+                                  X_a = X T' itself and T is not stored */
 } mda_letkf_params;
 
 enum { MDA_EIG_AUTO = 0, MDA_EIG_JACOBI = 1 };
@@ -315,7 +320,8 @@ int mda_letkf_analyse_batch(mda_ctx* ctx, int32_t nb, const mda_letkf_params* pr
 
 /* Per-column k x k transforms T (X_a = xbar + X' T) of the last LETKF call,
  * for the listed columns, row-major [ncols][k][k] (host).  Diagnostics and
- * parity tests; valid until the next call on the context.  Not kept when the
+ * parity tests; valid until the next call on the context.  Kept when that
+ * call set keep_transforms = 1 or ran a path that stores T; not kept when the
  * analysis ran its columns in passes (k > 64 on grids over 65,536 columns,
  * or a pass size set by the environment variable MDA_LETKF_CHUNK_COLS):
  * MDA_E_ARG then. */

@@ -83,7 +83,7 @@ class LetkfParams(C.Structure):
     _fields_ = [("inflation", _d), ("radius", _d), ("localization", _i32), ("rweight", _i32),
                 ("inflation_form", _i32), ("sqrt_form", _i32), ("combine", _i32),
                 ("precision", _i32), ("rho_column", C.c_void_p), ("rho_on_device", _i32),
-                ("eig_method", _i32), ("transform_precision", _i32)]
+                ("eig_method", _i32), ("transform_precision", _i32), ("keep_transforms", _i32)]
 
 
 class LetkfDiag(C.Structure):

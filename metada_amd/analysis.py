@@ -591,7 +591,7 @@ class LETKF(_AlgorithmBase):
     def __init__(self, ensemble, obs, obs_op, config: Config, geometry: ColumnGeometry = None,
                  ctx=None, form: str = "standard", precision: int = L.PREC_FP64,
                  rho_column=None, eig_method: int = L.EIG_AUTO,
-                 transform_precision: int = L.PREC_FP64):
+                 transform_precision: int = L.PREC_FP64, keep_transforms: bool = False):
         super().__init__(ensemble, obs, obs_op, ctx)
         self.inflation = config.Get("inflation").asFloat()
         self.localization_radius = config.Get("localization_radius").asFloat()
@@ -610,6 +610,9 @@ class LETKF(_AlgorithmBase):
         self.params.precision = precision
         self.params.eig_method = eig_method
         self.params.transform_precision = transform_precision
+        # transforms() needs T kept; otherwise the eigenvector kernel may apply
+        # X_a = X T' itself without storing T
+        self.params.keep_transforms = 1 if keep_transforms else 0
         self._rho_keep = None
         if rho_column is not None:
             rp, rdev, self._rho_keep = _buf(rho_column)

@@ -898,6 +898,9 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   // accumulation, or BF16x3 products (2^-16 relative) with FP32 accumulation
   sa.null_rel = prm->precision == MDA_PREC_FP32 ? 0x1p-22
                 : prm->precision == MDA_PREC_BF16X3 ? 0x1p-17 : 0.0;
+  // E3 applies the transform itself unless T is to be kept
+  sa.fuse = P.eig_path && letkf_eig_fusable(k) && prm->combine == MDA_COMBINE_MATRIX &&
+            prm->transform_precision == MDA_PREC_FP64 && !prm->keep_transforms;
   P.stats = c->get<int64_t>(pre + "col_stats", 9);
   return P;
 }
@@ -980,6 +983,10 @@ static letkf_solve_args solve_pass(const letkf_plan& P, int64_t c0, int64_t nc) 
   sa.de += c0 * 2 * P.k;
   sa.lam += c0 * P.k;
   sa.flag += c0;
+  sa.X = P.de.X;
+  sa.x_ncol = P.ncol;
+  sa.x_nlev = P.nlev;
+  sa.x_col0 = c0;
   return sa;
 }
 
@@ -1025,8 +1032,9 @@ static void enqueue_columns(hipStream_t s, letkf_plan& P, std::pair<int, int> cf
     for (int i = 0; i < 3; ++i) sa.ev_stage[i] = m ? m->e[2 + i] : nullptr;
     launch_letkf_solve(s, sa);
     if (m) MDA_HIP(hipEventRecord(m->e[5], s));
+    // fused: only the columns E3 left to the fallback remain
     launch_letkf_transform(s, P.de.X, P.k, P.ncol, P.nlev, P.sa.T, P.sa.status, c0, nc,
-                           P.prm->transform_precision);
+                           P.prm->transform_precision, sa.fuse ? sa.flag : nullptr);
     if (m) MDA_HIP(hipEventRecord(m->e[6], s));
   }
 }
@@ -1046,7 +1054,7 @@ static void finish_letkf(mda_ctx* c, letkf_plan& P, mda_letkf_diag* diag) {
   d2h(c, hs, P.stats, sizeof(hs));
   d2h(c, &hstat, P.sa.status, sizeof(int));
   // the transforms stay readable only when one pass covered every column
-  c->last_T = P.chunk == P.ncol ? P.sa.T : nullptr;
+  c->last_T = P.chunk == P.ncol && !P.sa.fuse ? P.sa.T : nullptr;
   c->last_k = P.k; c->last_ncol = P.ncol; c->last_nloc = P.ga.nloc;
   c->last_flag = P.eig_path ? P.sa.flag : nullptr;
   throw_status(hstat & MDA_ST_SKIP_UPDATE, "LETKF");
@@ -1378,7 +1386,8 @@ int mda_letkf_last_transforms(mda_ctx* c, const int64_t* cols, int64_t ncols, do
   return guarded(c, [&] {
     MDA_REQUIRE(c->last_nloc != nullptr, "no LETKF analysis on this context yet");
     MDA_REQUIRE(c->last_T != nullptr,
-                "the transforms of an analysis run in column passes are not kept");
+                "the transforms of that analysis were not kept (set keep_transforms = 1; "
+                "an analysis run in column passes never keeps them)");
     MDA_REQUIRE(cols && T_out, "cols and T_out are required");
     const size_t kk = (size_t)c->last_k * c->last_k;
     for (int64_t i = 0; i < ncols; ++i) {

@@ -16,6 +16,7 @@
 // re-solved by the parallel-order Jacobi kernel.
 #include "mda_device.hpp"
 #include "mda_internal.hpp"
+#include "mda_xform.hpp"
 
 namespace mda {
 
@@ -378,293 +379,331 @@ __device__ inline double fast_div(double a, double b) {
 
 constexpr int E3_SUB = 4;  // reflectors per static range in the back-transform
 
+// LDS of one column in E3, in doubles: Z, then sd, se, sl, sy, sb, swt (64
+// each) and the near-pair dot products [64][E3_BAND].
+constexpr int E3_BAND = 8;  // near pairs (j, j+1..j+BAND) of the sorted spectrum
 template <int N>
-__global__ __launch_bounds__(64) void k_trd_vectors(
-    int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
-    const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
-    const double* __restrict__ rho_col, int prior, int combine, double null_rel,
-    const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
+struct e3_shape {
+  static constexpr int NP = (N + 15) / 16 * 16;  // MFMA tile padding
+  static constexpr int ld = NP + 1;              // Z[i * ld + j]: component i of vector j
+  static constexpr int ZS = NP * ld > N * 64 ? NP * ld : N * 64;
+  static constexpr int SLICE = ZS + 6 * 64 + 64 * E3_BAND;
+};
+
+// E3 of column g by one wavefront, in the wavefront's own LDS slice of
+// e3_shape<N>::SLICE doubles.  FUSED = false: T of the column goes to T.
+// FUSED = true (N a multiple of 16, matrix combination): T stays in tb in
+// the MFMA B layout, tb[s][tj] = T[4s + kk][16 tj + i16], and nothing is
+// written to T; an empty column gets T = sqrt(rho) I there, and pre() is
+// called once the eigenvectors have left the registers (the caller issues
+// loads there whose latency the W products hide).  Returns true when tb
+// holds the column's T (FUSED), false for a column left to the Jacobi
+// fallback (flagged now or before) and, unfused, for an empty one.
+template <int N, bool FUSED, class Pre>
+__device__ __forceinline__ bool e3_column(
+    int64_t g, double* __restrict__ slice, const int* __restrict__ nloc,
+    const double* __restrict__ de, const double* __restrict__ lam,
+    const double* __restrict__ bvec, double rho, const double* __restrict__ rho_col, int prior,
+    int combine, double null_rel, const double* __restrict__ Ug, double* __restrict__ T,
+    int* __restrict__ flag, double (&tb)[e3_shape<N>::NP / 4][e3_shape<N>::NP / 16],
+    Pre&& pre) {
   constexpr int n = N;
-  constexpr int NP = (N + 15) / 16 * 16;  // MFMA tile padding
-  constexpr int ld = NP + 1;              // Z[i * ld + j]: component i of vector j
-  constexpr int ZS = NP * ld > N * 64 ? NP * ld : N * 64;
-  constexpr int BAND = 8;  // near pairs (j, j+1..j+BAND) of the sorted spectrum
+  constexpr int NP = e3_shape<N>::NP;
+  constexpr int ld = e3_shape<N>::ld;
+  constexpr int BAND = E3_BAND;
   // Z doubles as the per-lane U_i store [i * 64 + lane] of the factorisation
   // and as the staging area of the Householder rows for the back-transform.
-  __shared__ double Z[ZS];
-  __shared__ double sd[64], se[64], sl[64], sy[64], sb[64], swt[64];
-  __shared__ double dots[64][BAND];
-  const int lane = threadIdx.x;
-  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
-    if (nloc[g] == 0 || flag[g]) continue;  // empty columns / Jacobi fallback columns
-    const double r = rho_col ? rho_col[g] : rho;
-    double* Tg = T + (size_t)g * n * n;
-    const double* dg = de + (size_t)g * 2 * n;
-    sd[lane] = lane < n ? dg[lane] : 0.0;
-    se[lane] = lane < n - 1 ? dg[n + lane] : 0.0;
-    sl[lane] = lane < n ? lam[(size_t)g * n + lane] : 0.0;
-    sb[lane] = lane < n ? bvec[(size_t)g * n + lane] : 0.0;
-    wave_lds_sync();
-    double tnorm = 0.0;
+  double* Z = slice;
+  double* sd = slice + e3_shape<N>::ZS;
+  double* se = sd + 64;
+  double* sl = se + 64;
+  double* sy = sl + 64;
+  double* sb = sy + 64;
+  double* swt = sb + 64;
+  double* dots = swt + 64;  // [64][BAND]
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const double r = rho_col ? rho_col[g] : rho;
+  if (nloc[g] == 0) {
+    if constexpr (FUSED) {
+      const double sr = sqrt(r);
+#pragma unroll
+      for (int s4 = 0; s4 < NP / 4; ++s4)
+#pragma unroll
+        for (int tj = 0; tj < NP / 16; ++tj) tb[s4][tj] = 4 * s4 + kk == 16 * tj + i16 ? sr : 0.0;
+      return true;
+    }
+    return false;
+  }
+  if (flag[g]) return false;  // QL did not converge: Jacobi fallback
+  wave_lds_sync();            // the slice's previous readers are done
+  double* Tg = T + (size_t)g * n * n;
+  const double* dg = de + (size_t)g * 2 * n;
+  sd[lane] = lane < n ? dg[lane] : 0.0;
+  se[lane] = lane < n - 1 ? dg[n + lane] : 0.0;
+  sl[lane] = lane < n ? lam[(size_t)g * n + lane] : 0.0;
+  sb[lane] = lane < n ? bvec[(size_t)g * n + lane] : 0.0;
+  wave_lds_sync();
+  double tnorm = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if ((i & 7) == 0) lds_reload_point();
+    tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+  }
+  const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
+  const double l = sl[lane < n ? lane : 0];
+  // ---- eigenvector z of T for lambda_lane, in registers ----
+  // L_i built in place in z[], U_i parked in LDS, then
+  // z_kt = 1, z_i = -L_i z_{i+1} (i < kt), z_{i+1} = -U_i z_i (i >= kt)
+  double z[N];
+  double* Uv = Z;
+  {
+    double dp = sd[0] - l;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+      if ((i & 7) == 0) lds_reload_point();
+      z[i] = fast_div(se[i], safe_pivot(dp, tiny));  // L_i
+      dp = sd[i + 1] - l - z[i] * se[i];
+    }
+    z[N - 1] = 0.0;
+    double gbest = fabs(dp);  // gamma_{n-1} = D+_{n-1}
+    int kt = N - 1;
+    double dm = sd[N - 1] - l;
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) {
+      if ((i & 7) == 6) lds_reload_point();
+      const double u = fast_div(se[i], safe_pivot(dm, tiny));
+      Uv[i * 64 + lane] = u;
+      dm = sd[i] - l - u * se[i];
+      const double dpi = sd[i] - l - (i > 0 ? z[i > 0 ? i - 1 : 0] * se[i > 0 ? i - 1 : 0] : 0.0);
+      const double gam = fabs(dpi + dm - (sd[i] - l));
+      if (gam < gbest) { gbest = gam; kt = i; }
+    }
+    double zc = 1.0;
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i)
+      if (i < kt) { zc = -z[i] * zc; z[i] = zc; }
+    double nrm = 0.0;
+    zc = 1.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       if ((i & 7) == 0) lds_reload_point();
-      tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+      const double cand = i > 0 ? -Uv[(i > 0 ? i - 1 : 0) * 64 + lane] * zc : 0.0;
+      const double v = i < kt ? z[i] : (i == kt ? 1.0 : cand);
+      if (i >= kt) zc = v;
+      z[i] = lane < n ? v : 0.0;
+      nrm += z[i] * z[i];
     }
-    const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
-    const double l = sl[lane < n ? lane : 0];
-    // ---- eigenvector z of T for lambda_lane, in registers ----
-    // L_i built in place in z[], U_i parked in LDS, then
-    // z_kt = 1, z_i = -L_i z_{i+1} (i < kt), z_{i+1} = -U_i z_i (i >= kt)
-    double z[N];
-    double* Uv = Z;
-    {
-      double dp = sd[0] - l;
+    const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
 #pragma unroll
-      for (int i = 0; i < N - 1; ++i) {
-        if ((i & 7) == 0) lds_reload_point();
-        z[i] = fast_div(se[i], safe_pivot(dp, tiny));  // L_i
-        dp = sd[i + 1] - l - z[i] * se[i];
-      }
-      z[N - 1] = 0.0;
-      double gbest = fabs(dp);  // gamma_{n-1} = D+_{n-1}
-      int kt = N - 1;
-      double dm = sd[N - 1] - l;
+    for (int i = 0; i < N; ++i) z[i] *= inv;
+  }
+  wave_lds_sync();  // U_i reads done before Z is written
+  // ---- eigenpairs at sigma = (k-1)/rho (prior) or k-1 (posterior) ----
+  // W and wbar are formed around sigma:
+  //   W    = c [ f(sigma) I - sum_j g_j z_j z_j^T ],  g_j = sigma^-1/2 - lambda_j^-1/2
+  //   wbar = s [ b / sigma - sum_j (1/sigma - 1/lambda_j)(z_j^T b) z_j ]
+  // so eigenpairs with |lambda_j - sigma| <= 2e-12 sigma (the null space of
+  // Y'^T R^-1 Y', where eigenvectors are ill-determined) need no vector:
+  // each would change W by at most 1e-12 relative.  With a reduced-precision
+  // Gram the window widens to the Gram's own rounding level, null_rel ||A||.
+  const bool pri = prior != 0;
+  const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
+  const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
+  const bool skip = lane >= n || fabs(l - sigma) <= null_tol;
+  // weight of eigenpair j in W and wbar: 1 - sigma/lambda_j (0 at sigma)
+  const double wj = skip ? 0.0 : 1.0 - sigma / l;
+  sy[lane] = skip ? 1.0 : 0.0;  // skip flags, shared
+  swt[lane] = wj;
+  int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
+  if (!skip) {
+    double res = 0.0;
 #pragma unroll
-      for (int i = N - 2; i >= 0; --i) {
-        if ((i & 7) == 6) lds_reload_point();
-        const double u = fast_div(se[i], safe_pivot(dm, tiny));
-        Uv[i * 64 + lane] = u;
-        dm = sd[i] - l - u * se[i];
-        const double dpi = sd[i] - l - (i > 0 ? z[i > 0 ? i - 1 : 0] * se[i > 0 ? i - 1 : 0] : 0.0);
-        const double gam = fabs(dpi + dm - (sd[i] - l));
-        if (gam < gbest) { gbest = gam; kt = i; }
-      }
-      double zc = 1.0;
-#pragma unroll
-      for (int i = N - 2; i >= 0; --i)
-        if (i < kt) { zc = -z[i] * zc; z[i] = zc; }
-      double nrm = 0.0;
-      zc = 1.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        if ((i & 7) == 0) lds_reload_point();
-        const double cand = i > 0 ? -Uv[(i > 0 ? i - 1 : 0) * 64 + lane] * zc : 0.0;
-        const double v = i < kt ? z[i] : (i == kt ? 1.0 : cand);
-        if (i >= kt) zc = v;
-        z[i] = lane < n ? v : 0.0;
-        nrm += z[i] * z[i];
-      }
-      const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) z[i] *= inv;
+    for (int i = 0; i < N; ++i) {
+      if ((i & 7) == 0) lds_reload_point();
+      double v = (sd[i] - l) * z[i];
+      if (i > 0) v += se[i > 0 ? i - 1 : 0] * z[i > 0 ? i - 1 : 0];
+      if (i < N - 1) v += se[i] * z[i < N - 1 ? i + 1 : 0];
+      res = fmax(res, fabs(v));
     }
-    wave_lds_sync();  // U_i reads done before Z is written
-    // ---- eigenpairs at sigma = (k-1)/rho (prior) or k-1 (posterior) ----
-    // W and wbar are formed around sigma:
-    //   W    = c [ f(sigma) I - sum_j g_j z_j z_j^T ],  g_j = sigma^-1/2 - lambda_j^-1/2
-    //   wbar = s [ b / sigma - sum_j (1/sigma - 1/lambda_j)(z_j^T b) z_j ]
-    // so eigenpairs with |lambda_j - sigma| <= 2e-12 sigma (the null space of
-    // Y'^T R^-1 Y', where eigenvectors are ill-determined) need no vector:
-    // each would change W by at most 1e-12 relative.  With a reduced-precision
-    // Gram the window widens to the Gram's own rounding level, null_rel ||A||.
-    const bool pri = prior != 0;
-    const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
-    const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
-    const bool skip = lane >= n || fabs(l - sigma) <= null_tol;
-    // weight of eigenpair j in W and wbar: 1 - sigma/lambda_j (0 at sigma)
-    const double wj = skip ? 0.0 : 1.0 - sigma / l;
-    sy[lane] = skip ? 1.0 : 0.0;  // skip flags, shared
-    swt[lane] = wj;
-    int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
+    if (!(res <= 1e-12 * tnorm)) why |= 2;
+    if (!(l >= sigma - null_tol)) why |= 8;
+  }
+  if (lane < NP) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i];
+#pragma unroll
+    for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
+  }
+  wave_lds_sync();
+  // Close eigenpairs from twisted factorisations are accurate except for a
+  // small component of each along the other; a symmetric (Lowdin) step
+  // z_j -= 1/2 sum_q (z_j.z_q) z_q removes the non-orthogonality to second
+  // order and leaves only a rotation inside the pair, which changes W by
+  // (g_j - g_q) times that rotation: negligible for close eigenvalues.
+  bool any_near = false;
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+    for (int q = 0; q < BAND; ++q) dots[lane * BAND + q] = 0.0;
+    bool near = false;
     if (!skip) {
-      double res = 0.0;
+      for (int q = lane + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
+        if (sy[q] != 0.0) continue;
+        near = true;
+        double dot = 0.0;
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        if ((i & 7) == 0) lds_reload_point();
-        double v = (sd[i] - l) * z[i];
-        if (i > 0) v += se[i > 0 ? i - 1 : 0] * z[i > 0 ? i - 1 : 0];
-        if (i < N - 1) v += se[i] * z[i < N - 1 ? i + 1 : 0];
-        res = fmax(res, fabs(v));
+        for (int i = 0; i < N; ++i) dot += z[i] * Z[i * ld + q];
+        if (q - lane > BAND) {
+          if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;  // cluster wider than the band
+        } else {
+          dots[lane * BAND + q - lane - 1] = dot;
+          if (pass == 1 && fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
+        }
       }
-      if (!(res <= 1e-12 * tnorm)) why |= 2;
-      if (!(l >= sigma - null_tol)) why |= 8;
     }
+    wave_lds_sync();
+    if (pass == 0) any_near = __ballot(near) != 0ull;
+    if (pass == 1 || !any_near) break;
+    if (!skip) {
+      for (int q = lane - BAND; q <= lane + BAND; ++q) {
+        if (q < 0 || q >= n || q == lane) continue;
+        const double c = q > lane ? dots[lane * BAND + q - lane - 1] : dots[q * BAND + lane - q - 1];
+        if (c != 0.0) {
+#pragma unroll
+          for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Z[i * ld + q];
+        }
+      }
+    }
+    wave_lds_sync();
     if (lane < NP) {
 #pragma unroll
       for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i];
-#pragma unroll
-      for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
     }
     wave_lds_sync();
-    // Close eigenpairs from twisted factorisations are accurate except for a
-    // small component of each along the other; a symmetric (Lowdin) step
-    // z_j -= 1/2 sum_q (z_j.z_q) z_q removes the non-orthogonality to second
-    // order and leaves only a rotation inside the pair, which changes W by
-    // (g_j - g_q) times that rotation: negligible for close eigenvalues.
-    bool any_near = false;
-    for (int pass = 0; pass < 2; ++pass) {
+  }
+  for (int off = 32; off > 0; off >>= 1) why |= __shfl_xor(why, off);
+  if (why != 0) {
+    if (lane == 0) flag[g] = why;
+    wave_lds_sync();
+    return false;
+  }
+  // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
+  // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
+  // r = 0) are the same for every lane.  They are copied into the Z area
+  // (free between the near-pair step and U_s) by global_load_lds, 256
+  // bytes per instruction and no registers held, two chunks of 16 rows
+  // ahead of their use, and read back as LDS broadcasts.  Reflectors are
+  // applied in sub-chunks of E3_SUB so the static inner range starts at the
+  // sub-chunk's first non-zero position.
+  {
+    const double* Ucol = Ug + (size_t)g * n * n;
+    constexpr int NC = (N - 2 + 15) / 16;
+    // rows 16c .. min(16c + 15, N - 3) to Z + 16c N; the last instruction
+    // of a chunk may read up to 252 bytes past it, inside the column's
+    // n x n block (and inside Z)
+    auto stage = [&](int c) {
+      const int jlo = 16 * c;
+      const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
+      const int bytes = (jhi - jlo + 1) * N * 8;
+      const char* src = (const char*)(Ucol + (size_t)jlo * N) + 4 * lane;
+      char* dst = (char*)(Z + jlo * N);
+      for (int off = 0; off < bytes; off += 256)
+        __builtin_amdgcn_global_load_lds((const void*)(src + off),
+                                         (__attribute__((address_space(3))) void*)(dst + off),
+                                         4, 0, 0);
+    };
+    stage(NC - 1);
+    if (NC > 1) stage(NC - 2);
 #pragma unroll
-      for (int q = 0; q < BAND; ++q) dots[lane][q] = 0.0;
-      bool near = false;
-      if (!skip) {
-        for (int q = lane + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
-          if (sy[q] != 0.0) continue;
-          near = true;
-          double dot = 0.0;
+    for (int c = NC - 1; c >= 0; --c) {
+      const int jlo = 16 * c;
+      const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
+      // chunk c has landed once at most chunk c-1's copies (a full chunk:
+      // 16 N 8 / 256 = N / 2 instructions) are outstanding
+      if (c > 0) wait_vmcnt<N / 2>();
+      else wait_vmcnt<0>();
+      if (c >= 2) stage(c - 2);
+      // sub-chunks of E3_SUB reflectors, each with its own static range
 #pragma unroll
-          for (int i = 0; i < N; ++i) dot += z[i] * Z[i * ld + q];
-          if (q - lane > BAND) {
-            if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;  // cluster wider than the band
-          } else {
-            dots[lane][q - lane - 1] = dot;
-            if (pass == 1 && fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
-          }
-        }
-      }
-      wave_lds_sync();
-      if (pass == 0) any_near = __ballot(near) != 0ull;
-      if (pass == 1 || !any_near) break;
-      if (!skip) {
-        for (int q = lane - BAND; q <= lane + BAND; ++q) {
-          if (q < 0 || q >= n || q == lane) continue;
-          const double c = q > lane ? dots[lane][q - lane - 1] : dots[q][lane - q - 1];
-          if (c != 0.0) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Z[i * ld + q];
-          }
-        }
-      }
-      wave_lds_sync();
-      if (lane < NP) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i];
-      }
-      wave_lds_sync();
-    }
-    for (int off = 32; off > 0; off >>= 1) why |= __shfl_xor(why, off);
-    if (why != 0) {
-      if (lane == 0) flag[g] = why;
-      wave_lds_sync();
-      continue;
-    }
-    // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
-    // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
-    // r = 0) are the same for every lane.  They are copied into the Z area
-    // (free between the near-pair step and U_s) by global_load_lds, 256
-    // bytes per instruction and no registers held, two chunks of 16 rows
-    // ahead of their use, and read back as LDS broadcasts.  Reflectors are
-    // applied in sub-chunks of E3_SUB so the static inner range starts at the
-    // sub-chunk's first non-zero position.
-    {
-      const double* Ucol = Ug + (size_t)g * n * n;
-      constexpr int NC = (N - 2 + 15) / 16;
-      // rows 16c .. min(16c + 15, N - 3) to Z + 16c N; the last instruction
-      // of a chunk may read up to 252 bytes past it, inside the column's
-      // n x n block (and inside Z)
-      auto stage = [&](int c) {
-        const int jlo = 16 * c;
-        const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
-        const int bytes = (jhi - jlo + 1) * N * 8;
-        const char* src = (const char*)(Ucol + (size_t)jlo * N) + 4 * lane;
-        char* dst = (char*)(Z + jlo * N);
-        for (int off = 0; off < bytes; off += 256)
-          __builtin_amdgcn_global_load_lds((const void*)(src + off),
-                                           (__attribute__((address_space(3))) void*)(dst + off),
-                                           4, 0, 0);
-      };
-      stage(NC - 1);
-      if (NC > 1) stage(NC - 2);
-#pragma unroll
-      for (int c = NC - 1; c >= 0; --c) {
-        const int jlo = 16 * c;
-        const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
-        // chunk c has landed once at most chunk c-1's copies (a full chunk:
-        // 16 N 8 / 256 = N / 2 instructions) are outstanding
-        if (c > 0) wait_vmcnt<N / 2>();
-        else wait_vmcnt<0>();
-        if (c >= 2) stage(c - 2);
-        // sub-chunks of E3_SUB reflectors, each with its own static range
-#pragma unroll
-        for (int sc = 16 / E3_SUB - 1; sc >= 0; --sc) {
-          const int slo = jlo + E3_SUB * sc;
-          const int shi = (slo + E3_SUB - 1 < jhi) ? slo + E3_SUB - 1 : jhi;
-          if (slo > jhi) continue;
+      for (int sc = 16 / E3_SUB - 1; sc >= 0; --sc) {
+        const int slo = jlo + E3_SUB * sc;
+        const int shi = (slo + E3_SUB - 1 < jhi) ? slo + E3_SUB - 1 : jhi;
+        if (slo > jhi) continue;
 #pragma unroll 1
-          for (int j = shi; j >= slo; --j) {
-            const double* u = Z + j * N;
-            const double tau = u[0];
-            double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        for (int j = shi; j >= slo; --j) {
+          const double* u = Z + j * N;
+          const double tau = u[0];
+          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-            for (int i = slo + 1; i < N; ++i) {
-              const double t = u[i] * z[i];
-              switch (i & 3) {
-                case 0: p0 += t; break;
-                case 1: p1 += t; break;
-                case 2: p2 += t; break;
-                default: p3 += t; break;
-              }
+          for (int i = slo + 1; i < N; ++i) {
+            const double t = u[i] * z[i];
+            switch (i & 3) {
+              case 0: p0 += t; break;
+              case 1: p1 += t; break;
+              case 2: p2 += t; break;
+              default: p3 += t; break;
             }
-            const double dot = tau * ((p0 + p1) + (p2 + p3));
-#pragma unroll
-            for (int i = slo + 1; i < N; ++i) z[i] -= dot * u[i];
           }
+          const double dot = tau * ((p0 + p1) + (p2 + p3));
+#pragma unroll
+          for (int i = slo + 1; i < N; ++i) z[i] -= dot * u[i];
         }
       }
     }
-    // ---- U_s = Z diag(sqrt(g)) and y' for wbar ----
-    const double sc = pri ? 1.0 : r;
-    const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
-    double yb = 0.0;
-    if (!skip) {
+  }
+  // ---- U_s = Z diag(sqrt(g)) and y' for wbar ----
+  const double sc = pri ? 1.0 : r;
+  const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
+  double yb = 0.0;
+  if (!skip) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) yb += z[i] * sb[i];
-      yb *= (1.0 / sigma - 1.0 / l);
-    }
-    sy[lane] = f > 0.0 ? yb / f : 0.0;  // Z y = U_s (y / f)
-    if (lane < NP) {
+    for (int i = 0; i < N; ++i) yb += z[i] * sb[i];
+    yb *= (1.0 / sigma - 1.0 / l);
+  }
+  sy[lane] = f > 0.0 ? yb / f : 0.0;  // Z y = U_s (y / f)
+  if (lane < NP) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i] * f;
+    for (int i = 0; i < N; ++i) Z[i * ld + lane] = z[i] * f;
 #pragma unroll
-      for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
-    }
+    for (int i = N; i < NP; ++i) Z[i * ld + lane] = 0.0;
+  }
+  wave_lds_sync();
+  // wbar_r = s [ b_r / sigma - sum_j U_s[r][j] y'_j ]
+  double wbar = 0.0;
+  if (lane < n) {
+    double acc = 0.0;
+#pragma unroll
+    for (int e2 = 0; e2 < N; ++e2) acc += Z[lane * ld + e2] * sy[e2];
+    wbar = sc * (sb[lane] / sigma - acc);
+  }
+  const double coef = sqrt((double)(n - 1)) * sqrt(sc);
+  const double fs = 1.0 / sqrt(sigma);
+  if (combine == MDA_COMBINE_ELEMENTWISE) {
+    // t_i = wbar_i + coef (f(sigma) - sum_a U_s[i][a] (sum_j U_s[j][a]))
+    double cs = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) cs += z[i] * f;
     wave_lds_sync();
-    // wbar_r = s [ b_r / sigma - sum_j U_s[r][j] y'_j ]
-    double wbar = 0.0;
+    sy[lane] = cs;
+    wave_lds_sync();
+    double t = wbar;
     if (lane < n) {
       double acc = 0.0;
 #pragma unroll
-      for (int e2 = 0; e2 < N; ++e2) acc += Z[lane * ld + e2] * sy[e2];
-      wbar = sc * (sb[lane] / sigma - acc);
+      for (int a = 0; a < N; ++a) acc += Z[lane * ld + a] * sy[a];
+      t += coef * (fs - acc);
     }
-    const double coef = sqrt((double)(n - 1)) * sqrt(sc);
-    const double fs = 1.0 / sqrt(sigma);
-    if (combine == MDA_COMBINE_ELEMENTWISE) {
-      // t_i = wbar_i + coef (f(sigma) - sum_a U_s[i][a] (sum_j U_s[j][a]))
-      double cs = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) cs += z[i] * f;
-      wave_lds_sync();
-      sy[lane] = cs;
-      wave_lds_sync();
-      double t = wbar;
-      if (lane < n) {
-        double acc = 0.0;
-#pragma unroll
-        for (int a = 0; a < N; ++a) acc += Z[lane * ld + a] * sy[a];
-        t += coef * (fs - acc);
-      }
-      for (int idx = lane; idx < n * n; idx += 64) Tg[idx] = 0.0;
-      if (lane < n) Tg[(size_t)lane * n + lane] = t;
-      wave_lds_sync();
-      continue;
-    }
+    for (int idx = lane; idx < n * n; idx += 64) Tg[idx] = 0.0;
+    if (lane < n) Tg[(size_t)lane * n + lane] = t;
     wave_lds_sync();
-    sy[lane] = wbar;
-    wave_lds_sync();
-    // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA (lower tiles, mirrored) ----
-    constexpr int nt = NP / 16;
-    const int i16 = lane & 15, kk = lane >> 4;
+    return false;
+  }
+  wave_lds_sync();
+  sy[lane] = wbar;
+  wave_lds_sync();
+  // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA (lower tiles, mirrored) ----
+  constexpr int nt = NP / 16;
+  if constexpr (!FUSED) {
 #pragma unroll 1
     for (int ti = 0; ti < nt; ++ti) {
 #pragma unroll 1
@@ -680,15 +719,137 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
         for (int q = 0; q < 4; ++q) {
           const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
           if (row < n && col < n) {
-            const double w = coef * ((row == col ? fs : 0.0) - acc[q]);
-            Tg[(size_t)row * n + col] = sy[row] + w;
-            if (ti != tj) Tg[(size_t)col * n + row] = sy[col] + w;
+            // T = wbar + coef wd with one rounding (the fused path's form)
+            const double wd = (row == col ? fs : 0.0) - acc[q];
+            Tg[(size_t)row * n + col] = fma(coef, wd, sy[row]);
+            if (ti != tj) Tg[(size_t)col * n + row] = fma(coef, wd, sy[col]);
           }
         }
       }
     }
     wave_lds_sync();
+    return false;
+  } else {
+    static_assert(N % 16 == 0, "the fused transform needs N a multiple of 16");
+    pre();
+    // the lower tiles in registers until U_s has been read, then the full
+    // symmetric W / coef through Z and T = wbar 1^T + W into the B layout:
+    // the same values, rounded the same way, as the unfused path stores
+    d4 wt[nt * (nt + 1) / 2];
+#pragma unroll
+    for (int ti = 0; ti < nt; ++ti) {
+#pragma unroll
+      for (int tj = 0; tj <= ti; ++tj) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < NP / 4; ++s4) {
+          const double av = Z[(16 * ti + i16) * ld + 4 * s4 + kk];
+          const double bv = Z[(16 * tj + i16) * ld + 4 * s4 + kk];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+        wt[ti * (ti + 1) / 2 + tj] = acc;
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int ti = 0; ti < nt; ++ti) {
+#pragma unroll
+      for (int tj = 0; tj <= ti; ++tj) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
+          const double wd = (row == col ? fs : 0.0) - wt[ti * (ti + 1) / 2 + tj][q];
+          Z[row * ld + col] = wd;
+          if (ti != tj) Z[col * ld + row] = wd;
+        }
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int s4 = 0; s4 < NP / 4; ++s4)
+#pragma unroll
+      for (int tj = 0; tj < nt; ++tj) {
+        const int row = 4 * s4 + kk, col = 16 * tj + i16;
+        tb[s4][tj] = fma(coef, Z[row * ld + col], sy[row]);
+      }
+    wave_lds_sync();
+    return true;
   }
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void k_trd_vectors(
+    int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+    const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
+    const double* __restrict__ rho_col, int prior, int combine, double null_rel,
+    const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
+  // one column per block: no loop-carried state (a grid-stride loop here
+  // costs hundreds of SGPR spills)
+  __shared__ double slice[e3_shape<N>::SLICE];
+  double tb[e3_shape<N>::NP / 4][e3_shape<N>::NP / 16];
+  const int64_t g = blockIdx.x;
+  if (g < ncol)
+    e3_column<N, false>(g, slice, nloc, de, lam, bvec, rho, rho_col, prior, combine, null_rel, Ug,
+                        T, flag, tb, [] {});
+}
+
+constexpr int E3_PRE = 3;  // 16-level chunks of X loaded during E3's W products
+
+// E3 with the transform fused: a block of XM_C = 4 wavefronts takes 4
+// consecutive columns of the pass (one each, LDS slices side by side), and
+// once every wavefront holds its T the block applies X_a = X T' to the 4
+// columns (mda_xform.hpp) in place, so T never reaches HBM.  Columns left to
+// the Jacobi fallback are not updated here (the transform restricted to the
+// flagged columns does them after the fallback).  Blocks take groups in an
+// XCD-contiguous order (gridDim.x a multiple of 8), so the groups that share
+// X's cache lines run on one XCD at about the same time.
+template <int N>
+__global__ __launch_bounds__(256) void k_trd_vectors_x(
+    int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+    const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
+    const double* __restrict__ rho_col, int prior, double null_rel,
+    const double* __restrict__ Ug, int* __restrict__ flag, double* __restrict__ X,
+    int64_t x_ncol, int64_t nlev, int64_t col0, int* __restrict__ status) {
+  static_assert(N % 16 == 0 && N <= 64, "N must be a multiple of 16, at most 64");
+  constexpr int SL = e3_shape<N>::SLICE;
+  static_assert(XM_C * N * XM_L <= XM_C * SL, "X staging exceeds the block's LDS");
+  __shared__ double lds[XM_C * SL];
+  __shared__ int sok[XM_C];
+  const int tid = threadIdx.x, wv = tid >> 6;
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t g0 = vb * XM_C;  // within the pass
+  if (g0 >= ncol) return;         // block-uniform
+  const int ncl = (int)(ncol - g0 < XM_C ? ncol - g0 : XM_C);
+  double tb[N / 4][N / 16];
+#pragma unroll
+  for (int s4 = 0; s4 < N / 4; ++s4)
+#pragma unroll
+    for (int tj = 0; tj < N / 16; ++tj) tb[s4][tj] = 0.0;
+  // the first E3_PRE 16-level chunks of the block's X, each thread its
+  // share, loaded by every wavefront during its W products (or at once by a
+  // wavefront without them)
+  double pre[E3_PRE][xm_nv<N>()];
+  bool loaded = false;
+  auto preload = [&]() {
+    if (loaded) return;
+#pragma unroll
+    for (int ci = 0; ci < E3_PRE; ++ci)
+      xm_load<N>(pre[ci], X, x_ncol, nlev, col0 + g0, ncl, (int64_t)XM_L * ci);
+    loaded = true;
+  };
+  bool ok = false;
+  if (wv < ncl)
+    ok = e3_column<N, true>(g0 + wv, lds + wv * SL, nloc, de, lam, bvec, rho, rho_col, prior,
+                            MDA_COMBINE_MATRIX, null_rel, Ug, nullptr, flag, tb, preload);
+  preload();
+  if ((tid & 63) == 0) sok[wv] = ok ? 1 : 0;
+  __syncthreads();
+  unsigned store = 0u;
+  for (int c = 0; c < ncl; ++c) store |= sok[c] ? 1u << c : 0u;
+  if (store == 0u || (*status & MDA_ST_SKIP_UPDATE)) return;  // block-uniform
+  xm_tprime<N>(tb);
+  xm_apply<N, E3_PRE>(X, x_ncol, nlev, col0 + g0, ncl, store, tb, lds, status, pre);
 }
 
 // ---------------------------------------------------------------------------
@@ -1153,7 +1314,24 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
                      a.ncol, a.nloc, de, lam, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
-  hipLaunchKernelGGL(k_trd_vectors<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.nloc, de, lam, a.b,
+  if constexpr (N % 16 == 0) {
+    if (a.fuse) {
+      MDA_REQUIRE(a.combine == MDA_COMBINE_MATRIX && a.X && a.status,
+                  "fused eigenvector transform needs the matrix combination and X");
+      const int64_t ngrp = (a.ncol + XM_C - 1) / XM_C;
+      MDA_REQUIRE(ngrp < (int64_t)1 << 31, "eigenvector grid too large");
+      const unsigned nbx = (unsigned)((ngrp + 7) / 8 * 8);
+      hipLaunchKernelGGL(k_trd_vectors_x<N>, dim3(nbx), dim3(256), 0, s, a.ncol, a.nloc, de, lam,
+                         a.b, a.rho, a.rho_col, prior, a.null_rel, a.U, flag, a.X, a.x_ncol,
+                         a.x_nlev, a.x_col0, a.status);
+      MDA_LAUNCH_CHECK();
+      if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
+      return;
+    }
+  }
+  MDA_REQUIRE(!a.fuse, "fused eigenvector transform: no kernel for this k");
+  MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigenvector grid too large");
+  hipLaunchKernelGGL(k_trd_vectors<N>, dim3((unsigned)a.ncol), dim3(64), 0, s, a.ncol, a.nloc, de, lam, a.b,
                      a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.T, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
@@ -1163,6 +1341,7 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
 template <int N>
 static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, double* lam,
                          int* flag) {
+  MDA_REQUIRE(!a.fuse, "fused eigenvector transform: no kernel for this k");
   const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
   launch_zero(s, flag, sizeof(int) * a.ncol);
   const unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
@@ -1179,6 +1358,8 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));
 }
+
+bool letkf_eig_fusable(int k) { return k == 16 || k == 32 || k == 48 || k == 64; }
 
 bool letkf_eig_supported(int k) {
   switch (k) {

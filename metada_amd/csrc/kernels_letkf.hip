@@ -10,6 +10,7 @@
 #include "mda_crmath.hpp"
 #include "mda_device.hpp"
 #include "mda_internal.hpp"
+#include "mda_xform.hpp"
 
 namespace mda {
 
@@ -1738,121 +1739,47 @@ __global__ __launch_bounds__(256) void k_letkf_transform_reg(double* __restrict_
   }
 }
 
-// 4''. X_a = xbar (1 - colsum) + X' T as one FP64 MFMA product per column:
-// with T'[m][j] = T[m][j] + (1 - sum_m T[m][j]) / K,
-//   X_a[lev][j] = sum_m X[lev][m] T'[m][j]
-// (xbar = sum_m X[lev][m] / K), i.e. X_a = X T'.  A block covers XM_C = 4
-// consecutive columns, one wavefront each; the wavefront keeps T' of its
-// column in registers in the MFMA B layout (B[l/16][l%16], K/16 tiles x K/4
-// steps) and runs v_mfma_f64_16x16x4f64 over 16-level chunks.  X is staged
-// through LDS as [column][member][16 levels] from 32-byte column segments
-// (coalesced across the block's columns), and X_a leaves the same way.
-constexpr int XM_C = 4, XM_L = 16;
-
+// 4''. X_a = xbar (1 - colsum) + X' T as one FP64 MFMA product per column
+// (mda_xform.hpp): a block covers XM_C = 4 consecutive columns, one
+// wavefront each, T' of the column in registers.  `only` (optional, indexed
+// within the pass) restricts the update to the columns with only[g] != 0:
+// the columns the eigenvector kernel left to the Jacobi fallback.
 template <int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol,
-                                                              int64_t nlev,
-                                                              const double* __restrict__ T,
-                                                              int64_t col0, int64_t nc,
-                                                              int* __restrict__ status) {
+void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol, int64_t nlev,
+                            const double* __restrict__ T, int64_t col0, int64_t nc,
+                            int* __restrict__ status, const int* __restrict__ only) {
   static_assert(K % 16 == 0 && K <= 64, "K must be a multiple of 16, at most 64");
   if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int NTJ = K / 16, NS = K / 4;
   __shared__ double xs[XM_C * K * XM_L];  // [column][member or j][level]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i16 = lane & 15, kk = lane >> 4;
-  const int64_t n = ncol * nlev;
   const int64_t nb = gridDim.x;
   const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
   const int64_t g0 = vb * XM_C;  // within the pass [col0, col0 + nc)
   if (g0 >= nc) return;
   const int ncl = (int)(nc - g0 < XM_C ? nc - g0 : XM_C);
-  const bool own = wv < ncl;
-  // T' of column g0 + wv in the B layout: tb[s][tj] = T'[4s + kk][16 tj + i16]
+  unsigned store = (1u << ncl) - 1u;
+  if (only) {
+    unsigned m = 0u;
+    for (int c = 0; c < ncl; ++c) m |= only[g0 + c] != 0 ? 1u << c : 0u;
+    store = m;
+    if (store == 0u) return;  // block-uniform
+  }
+  const bool own = (store >> wv) & 1u;
+  // T of column g0 + wv in the B layout: tb[s][tj] = T[4s + kk][16 tj + i16]
   double tb[NS][NTJ];
   {
     const double* Tg = T + (size_t)(g0 + (own ? wv : 0)) * K * K;
-    double cs[NTJ];
-#pragma unroll
-    for (int tj = 0; tj < NTJ; ++tj) cs[tj] = 0.0;
 #pragma unroll
     for (int s4 = 0; s4 < NS; ++s4)
 #pragma unroll
-      for (int tj = 0; tj < NTJ; ++tj) {
+      for (int tj = 0; tj < NTJ; ++tj)
         tb[s4][tj] = own ? Tg[(size_t)(4 * s4 + kk) * K + 16 * tj + i16] : 0.0;
-        cs[tj] += tb[s4][tj];
-      }
-    const double rk = 1.0 / (double)K;
-#pragma unroll
-    for (int tj = 0; tj < NTJ; ++tj) {
-      cs[tj] += __shfl_xor(cs[tj], 16);
-      cs[tj] += __shfl_xor(cs[tj], 32);
-      const double add = (1.0 - cs[tj]) * rk;
-#pragma unroll
-      for (int s4 = 0; s4 < NS; ++s4) tb[s4][tj] += add;
-    }
   }
-  // X[m][l0 + l][g0 + c] of one chunk, NV values per thread; consecutive
-  // threads take consecutive columns of one (member, level) row segment.
-  // The next chunk's loads are in flight during this chunk's products and
-  // output.
-  constexpr int NV = XM_C * XM_L * K / 256;
-  double pv[NV];
-  auto load_chunk = [&](int64_t l0, int nll) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const int idx = tid + 256 * q;
-      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
-      double v = 0.0;
-      if (c < ncl && l < nll) v = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + c];
-      pv[q] = v;
-    }
-  };
-  load_chunk(0, (int)(nlev < XM_L ? nlev : XM_L));
-  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
-    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
-    __syncthreads();  // the previous chunk's output has left xs
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const int idx = tid + 256 * q;
-      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
-      xs[(c * K + m) * XM_L + l] = pv[q];
-    }
-    __syncthreads();
-    if (l0 + XM_L < nlev)
-      load_chunk(l0 + XM_L, (int)(nlev - l0 - XM_L < XM_L ? nlev - l0 - XM_L : XM_L));
-    d4 acc[NTJ];
-#pragma unroll
-    for (int tj = 0; tj < NTJ; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
-    const double* xa = xs + (wv * K + kk) * XM_L + i16;  // A[i16][kk] of step 0
-#pragma unroll
-    for (int s4 = 0; s4 < NS; ++s4) {
-      const double av = xa[4 * s4 * XM_L];
-#pragma unroll
-      for (int tj = 0; tj < NTJ; ++tj)
-        acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tb[s4][tj], acc[tj], 0, 0, 0);
-    }
-    __syncthreads();  // every wavefront has read its X
-    {
-      bool fin = true;
-#pragma unroll
-      for (int tj = 0; tj < NTJ; ++tj)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) fin = fin && isfinite(acc[tj][r]);
-      if (__ballot(!fin) != 0ull && lane == 0) atomicOr(status, MDA_ST_NONFINITE_STATE);
-    }
-    // D[kk + 4r][i16] of tile tj = X_a[level kk + 4r][j = 16 tj + i16] -> xs[c][j][l]
-#pragma unroll
-    for (int tj = 0; tj < NTJ; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) xs[(wv * K + 16 * tj + i16) * XM_L + kk + 4 * r] = acc[tj][r];
-    __syncthreads();
-    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
-      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, j = idx / (XM_C * XM_L);
-      if (c < ncl && l < nll) X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + c] = xs[(c * K + j) * XM_L + l];
-    }
-  }
+  xm_tprime<K>(tb);
+  xm_apply<K>(X, ncol, nlev, col0 + g0, ncl, store, tb, xs, status);
 }
 
 // 4'''. X_a = xbar + X' T with split-precision products, the transform of
@@ -2004,12 +1931,14 @@ bool letkf_transform_reg_supported(int k) {
 
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
                             const double* T, int* status, int64_t col0, int64_t nc,
-                            int precision) {
+                            int precision, const int* only) {
   MDA_REQUIRE(status != nullptr, "transform needs the analysis status word");
   MDA_REQUIRE(k <= 128, "transform supports k <= 128");
   if (nc < 0) nc = ncol;
   MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
   if (nc == 0) return;
+  MDA_REQUIRE(!only || (precision != MDA_PREC_BF16X3 && k % 16 == 0 && k <= 64),
+              "a column-restricted transform needs the FP64 MFMA path");
   if (precision == MDA_PREC_BF16X3 && k % 16 == 0 && k <= 64) {
     MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
@@ -2026,10 +1955,10 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
     MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
     switch (k) {
-      case 16: hipLaunchKernelGGL(k_letkf_transform_mfma<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
-      case 32: hipLaunchKernelGGL(k_letkf_transform_mfma<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
-      case 48: hipLaunchKernelGGL(k_letkf_transform_mfma<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
-      default: hipLaunchKernelGGL(k_letkf_transform_mfma<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status); break;
+      case 16: hipLaunchKernelGGL(k_letkf_transform_mfma<16>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status, only); break;
+      case 32: hipLaunchKernelGGL(k_letkf_transform_mfma<32>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status, only); break;
+      case 48: hipLaunchKernelGGL(k_letkf_transform_mfma<48>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status, only); break;
+      default: hipLaunchKernelGGL(k_letkf_transform_mfma<64>, dim3(nbm), dim3(256), 0, s, X, ncol, nlev, T, col0, nc, status, only); break;
     }
     MDA_LAUNCH_CHECK();
     return;

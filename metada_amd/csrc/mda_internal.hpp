@@ -333,7 +333,17 @@ struct letkf_solve_args {
   // Reduced-precision Gram: eigenvalues within null_rel * ||A|| of (k-1)/rho
   // are treated as the null space of Y'^T R^-1 Y' (0 for FP64)
   double null_rel;
+  // Fused eigenvectors + transform (tridiagonal path, k in {16, 32, 48, 64},
+  // matrix combination, FP64 transform): E3 applies X_a = X T' to the pass's
+  // columns [x_col0, x_col0 + ncol) of X (x_ncol columns of x_nlev levels)
+  // and writes T only for the columns it leaves to the fallback (flag != 0)
+  // and for the empty ones.
+  int fuse;
+  double* X;
+  int64_t x_ncol, x_nlev, x_col0;
 };
+// true when the tridiagonal eigensolver can apply the transform itself
+bool letkf_eig_fusable(int k);
 void launch_letkf_solve(hipStream_t s, const letkf_solve_args& a);
 // ensemble sizes with a compiled tridiagonal eigensolver (others use Jacobi)
 bool letkf_eig_supported(int k);
@@ -343,10 +353,11 @@ void launch_letkf_eig(hipStream_t s, const letkf_solve_args& a, double* de, doub
 // all; nc < 0: every column); T holds the nc columns' transforms.  Skipped
 // when the status word holds a MDA_ST_SKIP_UPDATE bit.  MDA_PREC_BF16X3
 // runs X' T as split-precision bf16 products with FP32 accumulation (k a
-// multiple of 16, at most 64); otherwise FP64.
+// multiple of 16, at most 64); otherwise FP64.  `only` ([nc], FP64 MFMA path
+// only) restricts the update to the columns with only[g] != 0.
 void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64_t nlev,
                             const double* T, int* status, int64_t col0 = 0, int64_t nc = -1,
-                            int precision = MDA_PREC_FP64);
+                            int precision = MDA_PREC_FP64, const int* only = nullptr);
 void launch_reduce_int(hipStream_t s, const int* v, int64_t n, int64_t* out2 /* sum,max */,
                        int64_t* out_empty);
 

@@ -128,7 +128,8 @@ def _letkf_run(mda, ctx, case, radius, rho, form="standard", device=False, cols_
                           coord=mda._lib.COORD_GEOGRAPHIC)
     geom = mda.ColumnGeometry(mda._lib.COORD_GEOGRAPHIC, case["col_lat"], case["col_lon"])
     cfg = mda.Config({"inflation": rho, "localization_radius": radius})
-    f = mda.LETKF(ens, obs, case["H"], cfg, geometry=geom, ctx=ctx, form=form)
+    f = mda.LETKF(ens, obs, case["H"], cfg, geometry=geom, ctx=ctx, form=form,
+                  keep_transforms=cols_T is not None)
     diag = f.Analyse()
     T = f.transforms(cols_T) if cols_T is not None else None
     return ens.to_host(), f, diag, T
@@ -425,13 +426,14 @@ def test_letkf_supplied_hx_matches_stencil(mda, ctx, orc, prec, bound):
         assert rms < bound, (name, rms)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
-def test_letkf_column_passes_match_single_pass(mda, ctx, orc, prec, monkeypatch):
+@pytest.mark.parametrize("prec,k", [("fp64", 40), ("bf16x3", 40), ("fp64", 64)])
+def test_letkf_column_passes_match_single_pass(mda, ctx, orc, prec, k, monkeypatch):
     """Gram -> solve -> transform over passes of 333 columns (the last one
     short) reproduces the single-pass analysis bit for bit: every column is
-    computed from the same inputs whatever pass it falls in."""
+    computed from the same inputs whatever pass it falls in (k = 64: the
+    eigenvector kernel applies the transform at the pass's column offset)."""
     code = {"fp64": mda._lib.PREC_FP64, "bf16x3": mda._lib.PREC_BF16X3}[prec]
-    case = latlon_case(k=40, p=4000, seed=91, cluster=True)
+    case = latlon_case(k=k, p=4000, seed=91, cluster=True)
     out = []
     for chunk in (None, "333"):
         if chunk is None:
