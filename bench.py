@@ -335,6 +335,12 @@ def main():
     e1 = nonempty * (4.0 / 3.0) * k ** 3
     e2 = nonempty * 30.0 * k * k
     e3 = nonempty * (4.0 * k ** 3 + 4.0 * k * k)
+    # with an FP64 transform at k in {16, 32, 48, 64} the eigenvector kernel
+    # applies X_a = X T' itself (2 n k FLOP, X in and out); the transform
+    # stage then covers only the columns left to the Jacobi fallback
+    fused = tp_value == L.PREC_FP64 and k in (16, 32, 48, 64)
+    if fused:
+        e3 += 2.0 * n * k
     # (kernel, bound, achieved, peak, unit, ms, what limits it)
     cand = {
         "gram": (gram_kernel, "mfma", gram_flop / (stages["ms_gram"] * 1e-3) / 1e12,
@@ -347,16 +353,21 @@ def main():
         "eig_values": ("k_trd_eigvals", "fp64", e2 / max(stages["ms_eig_values"], 1e-9) / 1e9,
                        PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_values"],
                        "FP64 VALU latency (serial QL recurrence per lane)"),
-        "eig_vectors": ("k_trd_vectors", "fp64", e3 / max(stages["ms_eig_vectors"], 1e-9) / 1e9,
+        "eig_vectors": ("k_trd_vectors_x" if fused else "k_trd_vectors", "fp64",
+                        e3 / max(stages["ms_eig_vectors"], 1e-9) / 1e9,
                         PEAK["fp64_TFs"], "TFLOP/s", stages["ms_eig_vectors"],
-                        "FP64 VALU issue and latency at one wave per SIMD"),
-        "transform": (("k_letkf_transform_bf16x3" if tp_value == L.PREC_BF16X3 else
-                       "k_letkf_transform_mfma") if k % 16 == 0 and k <= 64 else
-                      "k_letkf_transform_reg" if k in (8, 20, 40) else "k_letkf_transform", "hbm",
-                      xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
-                      PEAK["hbm_GBs"], "GB/s", stages["ms_transform"],
-                      "HBM access pattern (member-major X in 32-byte column segments)"),
+                        "FP64 VALU issue and latency at one wave per SIMD" +
+                        ("; X update in 32-byte column segments with the CU's four "
+                         "wavefronts waiting on it" if fused else "")),
     }
+    if not fused:
+        cand["transform"] = (
+            ("k_letkf_transform_bf16x3" if tp_value == L.PREC_BF16X3 else
+             "k_letkf_transform_mfma") if k % 16 == 0 and k <= 64 else
+            "k_letkf_transform_reg" if k in (8, 20, 40) else "k_letkf_transform", "hbm",
+            xform_bytes / (stages["ms_transform"] * 1e-3) / 1e9,
+            PEAK["hbm_GBs"], "GB/s", stages["ms_transform"],
+            "HBM access pattern (member-major X in 32-byte column segments)")
     dom = max(cand, key=lambda kk: cand[kk][5])
     kern, b, ach, pk, unit, _, limiter = cand[dom]
     # HBM bytes per launch of that kernel from the committed rocprofv3 FETCH_SIZE /
@@ -379,6 +390,11 @@ def main():
     stage_roofs = {key: {"kernel": v[0], "bound": v[1], "achieved": v[2], "peak": v[3],
                          "unit": v[4], "frac": v[2] / v[3], "ms": v[5]}
                    for key, v in cand.items()}
+    if fused:
+        # the X bytes the eigenvector kernel moves, against the HBM roof
+        stage_roofs["eig_vectors"]["hbm_GBs_algorithmic"] = (
+            xform_bytes / (stages["ms_eig_vectors"] * 1e-3) / 1e9)
+        stage_roofs["transform_fallback_columns"] = {"ms": stages["ms_transform"]}
     stage_roofs["obs_stats"] = {
         "kernel": "k_obs_stats (stage: tap-order sort + statistics)", "bound": "hbm",
         "achieved": obs_bytes / (stages["ms_obs"] * 1e-3) / 1e9, "peak": PEAK["hbm_GBs"],

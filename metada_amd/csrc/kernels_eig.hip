@@ -1027,275 +1027,329 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
   if (wave_any(bad) && lane == 0) flag[g] = 1;
 }
 
-// E3 for N = 128: thread j owns eigenvalue j and keeps its eigenvector in
-// registers.  The twisted factorisation stores D+_i in z[i] on the way down
-// and, in a second bottom-up pass, U_i in z[i+1] for i >= twist (slots not
-// needed by the top part).  Eigenvectors go through LDS (Z[component][vector],
-// row stride N + 1) for the near-pair (Lowdin) step and for
-// W = coef (f(sigma) I - U_s U_s^T), formed on FP64 MFMA by both wavefronts
-// (lower tiles dealt round-robin); one column per CU (129 KB of LDS).
-// Householder rows are staged through LDS 8 at a time.  Same checks and
-// fallback flags as the one-wavefront kernel.
+// Swaps a double between the two lanes of each even/odd pair (DPP
+// quad_perm [1, 0, 3, 2]).
+__device__ inline double pair_swap(double v) { return dpp_f64<0xB1>(v); }
+
+// E3 for N = 128 with four wavefronts per column.  Eigenvector j belongs to
+// the thread pair (2j, 2j + 1) of one wavefront; thread 2j + h holds the
+// components i = 2m + h (m < N/2) in registers, so the pair shares every dot
+// product and update evenly, nothing spills, and the block keeps all four
+// SIMDs of its CU busy (one column per CU: 155 KB of LDS).
+//   Twisted factorisation with its recurrences in Z (LDS): the even thread
+//   runs the top-down recurrence D+_i, the odd one the bottom-up D-_i, side
+//   by side; each stores the first half of its sequence and, over the
+//   second half, forms gamma_i = D+_i + D-_i - (d_i - l) from the partner's
+//   stored value.  After the twist kt is known, the one missing stretch of
+//   D+ or D- is recomputed, then the vector is built from kt downwards (even
+//   thread) and upwards (odd thread) at once.  Checks, near-pair (Lowdin)
+//   step, back-transform, W on FP64 MFMA and the T layout as in the
+//   one-wavefront kernel.
 template <int N>
-__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
-void k_trd_vectors_w(int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
-                     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
-                     const double* __restrict__ rho_col, int prior, int combine, double null_rel,
-                     const double* __restrict__ Ug, double* __restrict__ T,
-                     int* __restrict__ flag) {
-  static_assert(N == 128, "two wavefronts per column");
-  constexpr int n = N;
-  constexpr int ld = N + 1;
-  constexpr int BAND = 8;
-  constexpr int RC = 8;  // Householder rows per staging chunk
+__global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
+    int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
+    const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
+    const double* __restrict__ rho_col, int prior, int combine, double null_rel,
+    const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
+  static_assert(N == 128, "four wavefronts per column");
+  constexpr int n = N, ld = N + 1, BAND = 8, RC = 8, H = N / 2, M = N / 2;
   __shared__ double Z[N * ld];
   __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N], sk[N];
   __shared__ double dots[N][BAND];
   __shared__ double ur[RC * N];
   __shared__ int s_why, s_near;
-  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
-    // the thread index is re-read per column through an empty asm: addresses
-    // derived from it are then formed where they are used instead of being
-    // hoisted out of the column loop (several hundred live registers)
-    int t = threadIdx.x;
-    asm volatile("" : "+v"(t));
-    if (nloc[g] == 0 || flag[g]) continue;  // empty columns / fallback columns
-    const double r = rho_col ? rho_col[g] : rho;
-    double* Tg = T + (size_t)g * n * n;
-    const double* dg = de + (size_t)g * 2 * n;
+  const int t = threadIdx.x, j = t >> 1, h = t & 1;
+  const int64_t g = blockIdx.x;
+  if (g >= ncol || nloc[g] == 0 || flag[g]) return;  // block-uniform
+  const double r = rho_col ? rho_col[g] : rho;
+  double* Tg = T + (size_t)g * n * n;
+  const double* dg = de + (size_t)g * 2 * n;
+  if (t < n) {
     sd[t] = dg[t];
     se[t] = t < n - 1 ? dg[n + t] : 0.0;
     sl[t] = lam[(size_t)g * n + t];
     sb[t] = bvec[(size_t)g * n + t];
-    if (t == 0) { s_why = 0; s_near = 0; }
-    __syncthreads();
-    double tnorm = 0.0;
+  }
+  if (t == 0) { s_why = 0; s_near = 0; }
+  __syncthreads();
+  double tnorm = 0.0;
 #pragma unroll 8
-    for (int i = 0; i < N; ++i) tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
-    const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
-    const double l = sl[t];
-    double z[N];
-    {
-      // top-down: z[i] = D+_i (safeguarded), L_i = e_i / D+_i
-      double dp = sd[0] - l;
-#pragma unroll
-      for (int i = 0; i < N - 1; ++i) {
-        if ((i & 7) == 0) lds_reload_point();
-        z[i] = safe_pivot(dp, tiny);
-        dp = sd[i + 1] - l - fast_div(se[i], z[i]) * se[i];
+  for (int i = 0; i < N; ++i) tnorm = fmax(tnorm, fabs(sd[i]) + 2.0 * fabs(se[i]));
+  const double tiny = 1e-300 + 2.220446049250313e-16 * 1e-3 * tnorm;
+  const double l = sl[j];
+  double* Zj = Z + j;  // component i of vector j at Zj[i * ld]
+  // ---- twisted factorisation ----
+  // step s: i = s (h = 0, D+_i) or N-1-s (h = 1, D-_i),
+  //   x_i = (d_i - l) - e (e / safe(x_prev)),  e = e_{i-1} (down) or e_i (up)
+  double xp = 1.0, gbest = INFINITY;
+  int kt = N - 1;
+#pragma unroll 4
+  for (int s = 0; s < H; ++s) {
+    const int i = h ? N - 1 - s : s;
+    const double ee = h ? se[i] : (i > 0 ? se[i > 0 ? i - 1 : 0] : 0.0);
+    const double x = (sd[i] - l) - ee * fast_div(ee, safe_pivot(xp, tiny));
+    Zj[i * ld] = x;
+    xp = x;
+  }
+  wave_lds_sync();  // the partner's first halves are in Z
+#pragma unroll 4
+  for (int s = H; s < N; ++s) {
+    const int i = h ? N - 1 - s : s;
+    const double ee = h ? se[i] : se[i > 0 ? i - 1 : 0];
+    const double x = (sd[i] - l) - ee * fast_div(ee, safe_pivot(xp, tiny));
+    const double gam = fabs(x + Zj[i * ld] - (sd[i] - l));
+    // ties go to the higher index (i ascends for h = 0, descends for h = 1)
+    if (h ? gam < gbest : gam <= gbest) { gbest = gam; kt = i; }
+    xp = x;
+  }
+  {
+    const double og = pair_swap(gbest);
+    const int ok = __builtin_amdgcn_mov_dpp(kt, 0xB1, 0xf, 0xf, true);
+    if (og < gbest || (og == gbest && ok > kt)) { gbest = og; kt = ok; }
+  }
+  // the missing stretch: D-_i for kt < i < H (bottom-up from D-_H), or D+_i
+  // for H <= i < kt (top-down from D+_{H-1}), by the even thread, over slots
+  // whose stored values are no longer needed
+  {
+    const bool up = kt < H;
+    double xq = Zj[(up ? H : H - 1) * ld];
+    for (int r2 = 0; r2 < H; ++r2) {
+      const int i = up ? H - 1 - r2 : H + r2;
+      const bool act = h == 0 && (up ? i >= kt + 1 : i <= kt - 1);
+      if (!wave_any(act)) break;
+      if (act) {
+        const double ee = up ? se[i] : se[i > 0 ? i - 1 : 0];
+        const double x = (sd[i] - l) - ee * fast_div(ee, safe_pivot(xq, tiny));
+        Zj[i * ld] = x;
+        xq = x;
       }
-      z[N - 1] = dp;
-      double gbest = fabs(dp);
-      int kt = N - 1;
-      double dm = sd[N - 1] - l;
-#pragma unroll
-      for (int i = N - 2; i >= 0; --i) {
-        if ((i & 7) == 6) lds_reload_point();
-        const double u = fast_div(se[i], safe_pivot(dm, tiny));
-        dm = sd[i] - l - u * se[i];
-        const double gam = fabs(z[i] + dm - (sd[i] - l));
-        if (gam < gbest) { gbest = gam; kt = i; }
-      }
-      // U_i (i >= kt) into z[i+1], by the same bottom-up recurrence.  The
-      // shift goes through an empty asm so the compiler recomputes the
-      // recurrence instead of keeping all of the first pass's U_i live.
-      double lr = l;
-      asm volatile("" : "+v"(lr));
-      dm = sd[N - 1] - lr;
-#pragma unroll
-      for (int i = N - 2; i >= 0; --i) {
-        if ((i & 7) == 6) lds_reload_point();
-        const double u = fast_div(se[i], safe_pivot(dm, tiny));
-        dm = sd[i] - lr - u * se[i];
-        if (i >= kt) z[i + 1] = u;
-      }
-      double zc = 1.0;
-#pragma unroll
-      for (int i = N - 2; i >= 0; --i) {
-        if ((i & 7) == 6) lds_reload_point();
-        if (i < kt) { zc = -fast_div(se[i], z[i]) * zc; z[i] = zc; }  // z_i = -L_i z_{i+1}
-      }
-      double nrm = 0.0;
-      zc = 1.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double v = i < kt ? z[i] : (i == kt ? 1.0 : -z[i] * zc);
-        if (i >= kt) zc = v;
-        z[i] = v;
-        nrm += v * v;
-      }
-      const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) z[i] *= inv;
     }
-    const bool pri = prior != 0;
-    const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
-    const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
-    const bool skip = fabs(l - sigma) <= null_tol;
-    const double wj = skip ? 0.0 : 1.0 - sigma / l;
-    sy[t] = skip ? 1.0 : 0.0;  // skip flags, shared
-    swt[t] = wj;
-    int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
-    if (!skip) {
-      double res = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        if ((i & 7) == 0) lds_reload_point();
-        double v = (sd[i] - l) * z[i];
-        if (i > 0) v += se[i > 0 ? i - 1 : 0] * z[i > 0 ? i - 1 : 0];
-        if (i < N - 1) v += se[i] * z[i < N - 1 ? i + 1 : 0];
-        res = fmax(res, fabs(v));
+  }
+  wave_lds_sync();
+  // z_kt = 1; even thread downwards z_i = -(e_i / safe(D+_i)) z_{i+1},
+  // odd thread upwards z_{i+1} = -(e_i / safe(D-_{i+1})) z_i
+  if (h == 0) Zj[kt * ld] = 1.0;
+  {
+    double zc = 1.0;
+    for (int r2 = 0; r2 < N - 1; ++r2) {
+      const int i = h ? kt + r2 : kt - 1 - r2;
+      const bool act = h ? i <= N - 2 : i >= 0;
+      if (!wave_any(act)) break;
+      if (act) {
+        const int row = h ? i + 1 : i;
+        zc = -fast_div(se[i], safe_pivot(Zj[row * ld], tiny)) * zc;
+        Zj[row * ld] = zc;
       }
+    }
+  }
+  wave_lds_sync();
+  double z[M];
+  {
+    double nrm = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      z[m] = Zj[(2 * m + h) * ld];
+      nrm += z[m] * z[m];
+    }
+    nrm += pair_swap(nrm);
+    const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) z[m] *= inv;
+  }
+  const bool pri = prior != 0;
+  const double sigma = pri ? (double)(n - 1) / r : (double)(n - 1);
+  const double null_tol = fmax(2e-12 * sigma, null_rel * tnorm);
+  const bool skip = fabs(l - sigma) <= null_tol;
+  const double wj = skip ? 0.0 : 1.0 - sigma / l;
+  int why = 0;  // 2: residual, 4: near-pair orthogonality, 8: lambda < sigma
+  {
+    // residual (T - l I) z from the pair's registers: the neighbours of
+    // component 2m + h are the partner's m - 1 + h and m + h
+    double res = 0.0, pprev = 0.0, pcur = pair_swap(z[0]);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const double pnext = m + 1 < M ? pair_swap(z[m + 1 < M ? m + 1 : m]) : 0.0;
+      const int i = 2 * m + h;
+      const double left = h ? pcur : pprev;    // component i - 1
+      const double right = h ? pnext : pcur;   // component i + 1
+      double v = (sd[i] - l) * z[m];
+      if (i > 0) v += se[i > 0 ? i - 1 : 0] * left;
+      v += se[i] * right;  // se[N - 1] = 0
+      res = fmax(res, fabs(v));
+      pprev = pcur;
+      pcur = pnext;
+    }
+    res = fmax(res, pair_swap(res));
+    if (!skip) {
       if (!(res <= 1e-12 * tnorm)) why |= 2;
       if (!(l >= sigma - null_tol)) why |= 8;
     }
+  }
+  if (h == 0) {
+    sy[j] = skip ? 1.0 : 0.0;  // skip flags, shared
+    swt[j] = wj;
+  }
 #pragma unroll
-    for (int i = 0; i < N; ++i) Z[i * ld + t] = z[i];
-    __syncthreads();
-    // near pairs: symmetric (Lowdin) correction, then a check
-    for (int pass = 0; pass < 2; ++pass) {
+  for (int m = 0; m < M; ++m) Zj[(2 * m + h) * ld] = z[m];
+  __syncthreads();
+  // near pairs: symmetric (Lowdin) correction, then a check
+  for (int pass = 0; pass < 2; ++pass) {
+    if (h == 0) {
 #pragma unroll
-      for (int q = 0; q < BAND; ++q) dots[t][q] = 0.0;
-      bool near = false;
-      if (!skip) {
-        for (int q = t + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
-          if (sy[q] != 0.0) continue;
-          near = true;
-          double dot = 0.0;
-#pragma unroll
-          for (int i = 0; i < N; ++i) dot += z[i] * Z[i * ld + q];
-          if (q - t > BAND) {
-            if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
-          } else {
-            dots[t][q - t - 1] = dot;
-            if (pass == 1 && fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
-          }
-        }
-      }
-      if (pass == 0 && near) atomicOr(&s_near, 1);
-      __syncthreads();
-      if (pass == 1 || s_near == 0) break;
-      if (!skip) {
-        for (int q = t - BAND; q <= t + BAND; ++q) {
-          if (q < 0 || q >= n || q == t) continue;
-          const double c = q > t ? dots[t][q - t - 1] : dots[q][t - q - 1];
-          if (c != 0.0) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) z[i] -= 0.5 * c * Z[i * ld + q];
-          }
-        }
-      }
-      __syncthreads();  // every read of the uncorrected vectors is done
-#pragma unroll
-      for (int i = 0; i < N; ++i) Z[i * ld + t] = z[i];
-      __syncthreads();
+      for (int q = 0; q < BAND; ++q) dots[j][q] = 0.0;
     }
-    if (why) atomicOr(&s_why, why);
-    __syncthreads();
-    if (s_why != 0) {
-      if (t == 0) flag[g] = s_why;
-      __syncthreads();
-      continue;
-    }
-    // ---- z <- Q z, Householder rows staged through LDS ----
-    {
-      const double* Ucol = Ug + (size_t)g * n * n;
-      constexpr int NC = (N - 2 + RC - 1) / RC;
-#pragma unroll
-      for (int c = NC - 1; c >= 0; --c) {
-        const int jlo = RC * c;
-        const int jhi = (RC * c + RC - 1 < N - 3) ? RC * c + RC - 1 : N - 3;
-        __syncthreads();  // previous chunk's readers are done
-        for (int idx = t; idx < (jhi - jlo + 1) * N; idx += N) ur[idx] = Ucol[(size_t)jlo * N + idx];
-        __syncthreads();
-#pragma unroll 1
-        for (int j = jhi; j >= jlo; --j) {
-          const double* u = ur + (j - jlo) * N;
-          const double tau = u[0];
-          double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-#pragma unroll
-          for (int i = jlo + 1; i < N; ++i) {
-            if ((i & 15) == 0) lds_reload_point();
-            const double tt = u[i] * z[i];
-            switch (i & 3) {
-              case 0: p0 += tt; break;
-              case 1: p1 += tt; break;
-              case 2: p2 += tt; break;
-              default: p3 += tt; break;
-            }
-          }
-          const double dot = tau * ((p0 + p1) + (p2 + p3));
-#pragma unroll
-          for (int i = jlo + 1; i < N; ++i) {
-            if ((i & 15) == 0) lds_reload_point();
-            z[i] -= dot * u[i];
-          }
-        }
-      }
-    }
-    // ---- U_s = Z diag(f), y' for wbar ----
-    const double sc = pri ? 1.0 : r;
-    const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
-    double yb = 0.0;
+    bool near = false;
     if (!skip) {
+      for (int q = j + 1; q < n && (sl[q] - l) < 1e-3 * tnorm; ++q) {
+        if (sy[q] != 0.0) continue;
+        near = true;
+        double dot = 0.0;
 #pragma unroll
-      for (int i = 0; i < N; ++i) yb += z[i] * sb[i];
-      yb *= (1.0 / sigma - 1.0 / l);
+        for (int m = 0; m < M; ++m) dot += z[m] * Z[(2 * m + h) * ld + q];
+        dot += pair_swap(dot);
+        if (q - j > BAND) {
+          if (fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
+        } else {
+          if (h == 0) dots[j][q - j - 1] = dot;
+          if (pass == 1 && fabs(dot) * (wj + swt[q]) > 1e-12) why |= 4;
+        }
+      }
     }
-    sy[t] = f > 0.0 ? yb / f : 0.0;
-    double cs = 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      Z[i * ld + t] = z[i] * f;
-      cs += z[i] * f;
-    }
-    sk[t] = cs;
+    if (pass == 0 && near) atomicOr(&s_near, 1);
     __syncthreads();
-    const double coef = sqrt((double)(n - 1)) * sqrt(sc);
-    const double fs = 1.0 / sqrt(sigma);
-    // wbar_t = s [ b_t / sigma - sum_j U_s[t][j] y'_j ]
-    double acc = 0.0, acs = 0.0;
-    for (int e2 = 0; e2 < N; ++e2) {
-      const double us = Z[t * ld + e2];
-      acc += us * sy[e2];
-      acs += us * sk[e2];
+    if (pass == 1 || s_near == 0) break;
+    if (!skip) {
+      for (int q = j - BAND; q <= j + BAND; ++q) {
+        if (q < 0 || q >= n || q == j) continue;
+        const double c = q > j ? dots[j][q - j - 1] : dots[q][j - q - 1];
+        if (c != 0.0) {
+#pragma unroll
+          for (int m = 0; m < M; ++m) z[m] -= 0.5 * c * Z[(2 * m + h) * ld + q];
+        }
+      }
     }
-    const double wbar = sc * (sb[t] / sigma - acc);
+    __syncthreads();  // every read of the uncorrected vectors is done
+#pragma unroll
+    for (int m = 0; m < M; ++m) Zj[(2 * m + h) * ld] = z[m];
+    __syncthreads();
+  }
+  if (why) atomicOr(&s_why, why);
+  __syncthreads();
+  if (s_why != 0) {
+    if (t == 0) flag[g] = s_why;
+    return;  // block-uniform
+  }
+  // ---- z <- Q z, Householder rows staged through LDS RC at a time ----
+  {
+    const double* Ucol = Ug + (size_t)g * n * n;
+    constexpr int NC = (N - 2 + RC - 1) / RC;
+#pragma unroll
+    for (int c = NC - 1; c >= 0; --c) {
+      const int jlo = RC * c;
+      const int jhi = (RC * c + RC - 1 < N - 3) ? RC * c + RC - 1 : N - 3;
+      __syncthreads();  // previous chunk's readers are done
+      for (int idx = t; idx < (jhi - jlo + 1) * N; idx += 2 * N)
+        ur[idx] = Ucol[(size_t)jlo * N + idx];
+      __syncthreads();
+      // components i >= jlo + 1 take part: m >= jlo / 2, except i = jlo
+      const int m0 = jlo / 2;
+#pragma unroll 1
+      for (int jj = jhi; jj >= jlo; --jj) {
+        const double* u = ur + (jj - jlo) * N;
+        const double tau = u[0];
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+        for (int m = m0; m < M; ++m) {
+          if ((m & 7) == 0) lds_reload_point();
+          const double uu = (m == m0 && h == 0) ? 0.0 : u[2 * m + h];
+          const double tt = uu * z[m];
+          switch (m & 3) {
+            case 0: p0 += tt; break;
+            case 1: p1 += tt; break;
+            case 2: p2 += tt; break;
+            default: p3 += tt; break;
+          }
+        }
+        double dot = (p0 + p1) + (p2 + p3);
+        dot = tau * (dot + pair_swap(dot));
+#pragma unroll
+        for (int m = m0; m < M; ++m) {
+          if ((m & 7) == 0) lds_reload_point();
+          const double uu = (m == m0 && h == 0) ? 0.0 : u[2 * m + h];
+          z[m] -= dot * uu;
+        }
+      }
+    }
+  }
+  // ---- U_s = Z diag(f), y' for wbar ----
+  const double sc = pri ? 1.0 : r;
+  const double f = skip ? 0.0 : sqrt(1.0 / sqrt(sigma) - 1.0 / sqrt(l));
+  {
+    double yb = 0.0, cs = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      yb += z[m] * sb[2 * m + h];
+      Zj[(2 * m + h) * ld] = z[m] * f;
+      cs += z[m] * f;
+    }
+    yb += pair_swap(yb);
+    cs += pair_swap(cs);
+    if (skip) yb = 0.0;
+    else yb *= (1.0 / sigma - 1.0 / l);
+    if (h == 0) {
+      sy[j] = f > 0.0 ? yb / f : 0.0;
+      sk[j] = cs;
+    }
+  }
+  __syncthreads();
+  const double coef = sqrt((double)(n - 1)) * sqrt(sc);
+  const double fs = 1.0 / sqrt(sigma);
+  // wbar_j = s [ b_j / sigma - sum_e U_s[j][e] y'_e ], row j by its pair
+  double wbar;
+  {
+    double acc = 0.0, acs = 0.0;
+#pragma unroll 8
+    for (int m = 0; m < M; ++m) {
+      const double us = Z[j * ld + 2 * m + h];
+      acc += us * sy[2 * m + h];
+      acs += us * sk[2 * m + h];
+    }
+    acc += pair_swap(acc);
+    acs += pair_swap(acs);
+    wbar = sc * (sb[j] / sigma - acc);
     if (combine == MDA_COMBINE_ELEMENTWISE) {
       // t_i = wbar_i + coef (f(sigma) - sum_a U_s[i][a] (sum_j U_s[j][a]))
-      for (int idx = t; idx < n * n; idx += N) Tg[idx] = 0.0;
+      for (int idx = t; idx < n * n; idx += 2 * N) Tg[idx] = 0.0;
       __syncthreads();
-      Tg[(size_t)t * n + t] = wbar + coef * (fs - acs);
-      __syncthreads();
-      continue;
+      if (h == 0) Tg[(size_t)j * n + j] = wbar + coef * (fs - acs);
+      return;
     }
-    __syncthreads();  // sy (y') reads done
-    sy[t] = wbar;
-    __syncthreads();
-    // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA ----
-    constexpr int nt = N / 16;
-    const int lane = t & 63, wv = t >> 6;
-    const int i16 = lane & 15, kk = lane >> 4;
-    int tile = 0;
-    for (int ti = 0; ti < nt; ++ti) {
-      for (int tj = 0; tj <= ti; ++tj, ++tile) {
-        if ((tile & 1) != wv) continue;
-        d4 am = {0.0, 0.0, 0.0, 0.0};
-        const double* ra = Z + (16 * ti + i16) * ld + kk;
-        const double* rb = Z + (16 * tj + i16) * ld + kk;
+  }
+  __syncthreads();  // sy (y') reads done
+  if (h == 0) sy[j] = wbar;
+  __syncthreads();
+  // ---- W = coef (f(sigma) I - U_s U_s^T) on FP64 MFMA, tiles round-robin ----
+  constexpr int nt = N / 16;
+  const int lane = t & 63, wv = t >> 6;
+  const int i16 = lane & 15, kk = lane >> 4;
+  int tile = 0;
+  for (int ti = 0; ti < nt; ++ti) {
+    for (int tj = 0; tj <= ti; ++tj, ++tile) {
+      if ((tile & 3) != wv) continue;
+      d4 am = {0.0, 0.0, 0.0, 0.0};
+      const double* ra = Z + (16 * ti + i16) * ld + kk;
+      const double* rb = Z + (16 * tj + i16) * ld + kk;
 #pragma unroll 8
-        for (int s4 = 0; s4 < N / 4; ++s4)
-          am = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[4 * s4], rb[4 * s4], am, 0, 0, 0);
+      for (int s4 = 0; s4 < N / 4; ++s4)
+        am = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[4 * s4], rb[4 * s4], am, 0, 0, 0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
-          const double w = coef * ((row == col ? fs : 0.0) - am[q]);
-          Tg[(size_t)row * n + col] = sy[row] + w;
-          if (ti != tj) Tg[(size_t)col * n + row] = sy[col] + w;
-        }
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * ti + kk + 4 * q, col = 16 * tj + i16;
+        const double wd = (row == col ? fs : 0.0) - am[q];
+        Tg[(size_t)row * n + col] = fma(coef, wd, sy[row]);
+        if (ti != tj) Tg[(size_t)col * n + row] = fma(coef, wd, sy[col]);
       }
     }
-    __syncthreads();
   }
 }
 
@@ -1353,7 +1407,8 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
                      s, a.ncol, a.nloc, de, lam, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[1]) MDA_HIP(hipEventRecord(a.ev_stage[1], s));
-  hipLaunchKernelGGL(k_trd_vectors_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.nloc, de, lam, a.b,
+  MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigenvector grid too large");
+  hipLaunchKernelGGL(k_trd_vectors_w4<N>, dim3((unsigned)a.ncol), dim3(2 * N), 0, s, a.ncol, a.nloc, de, lam, a.b,
                      a.rho, a.rho_col, prior, a.combine, a.null_rel, a.U, a.T, flag);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[2]) MDA_HIP(hipEventRecord(a.ev_stage[2], s));

@@ -287,17 +287,18 @@ def test_letkf_rank_deficient_gram_needs_no_fallback(mda, ctx, orc):
     assert rel_err(ens.data, Xo) < TOL
 
 
-def test_letkf_repeated_eigenvalue_falls_back_to_jacobi(mda, ctx, orc):
+@pytest.mark.parametrize("k", [8, 128])
+def test_letkf_repeated_eigenvalue_falls_back_to_jacobi(mda, ctx, orc, k):
     """Two co-located observations whose anomaly rows are orthogonal with equal
     norms give A a double eigenvalue away from (k-1)/rho: the twisted-
     factorisation vectors coincide, the orthogonality check flags the column
-    and the Jacobi kernel re-solves it."""
-    k = 8
+    and the Jacobi kernel re-solves it (k = 128: the four-wavefront
+    eigenvector kernel and the Jacobi kernel in global scratch)."""
     rng = np.random.default_rng(2)
     X = rng.standard_normal((k, 3))
     hx = np.zeros((k, 2))
-    hx[:, 0] = [1, -1, 0, 0, 0, 0, 0, 0]
-    hx[:, 1] = [0, 0, 1, -1, 0, 0, 0, 0]
+    hx[0:2, 0] = [1, -1]
+    hx[2:4, 1] = [1, -1]
     yo = np.array([0.3, -0.2])
     obs = mda.Observation(yo, np.full(2, 0.25), np.zeros(2), np.zeros(2), np.zeros(2))
     ens = mda.Ensemble(X.copy(), 1, 1, 3)
