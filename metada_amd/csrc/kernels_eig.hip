@@ -99,8 +99,11 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
   constexpr int n = N;
   __shared__ double row[EIG_N], ub[EIG_N], wb[EIG_N];
   const int lane = threadIdx.x;
-  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
-    if (nloc[g] == 0) continue;
+  // one column per block: no loop-carried state (a grid-stride loop costs
+  // SGPR spills here)
+  {
+    const int64_t g = blockIdx.x;
+    if (g >= ncol || nloc[g] == 0) return;
     const double r = rho_col ? rho_col[g] : rho;
     const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
     const double* Cg = C + (size_t)g * n * n;
@@ -881,8 +884,9 @@ void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__
   constexpr int n = N;
   __shared__ double ub[N], wb[N], red[2][2], xs[2];
   const int t = threadIdx.x;
-  for (int64_t g = blockIdx.x; g < ncol; g += gridDim.x) {
-    if (nloc[g] == 0) continue;
+  {  // one column per block
+    const int64_t g = blockIdx.x;
+    if (g >= ncol || nloc[g] == 0) return;
     const double r = rho_col ? rho_col[g] : rho;
     const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
     const double* Cg = C + (size_t)g * n * n;
@@ -1359,7 +1363,8 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
                          int* flag) {
   const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
   launch_zero(s, flag, sizeof(int) * a.ncol);
-  const unsigned nb = (unsigned)(a.ncol < 131072 ? a.ncol : 131072);
+  MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigensolver grid too large");
+  const unsigned nb = (unsigned)a.ncol;  // one column per block
   hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
                      prior, de, a.U);
   MDA_LAUNCH_CHECK();
@@ -1398,7 +1403,8 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
   MDA_REQUIRE(!a.fuse, "fused eigenvector transform: no kernel for this k");
   const int prior = a.inflation_form == MDA_INFLFORM_PRIOR;
   launch_zero(s, flag, sizeof(int) * a.ncol);
-  const unsigned nb = (unsigned)(a.ncol < 65536 ? a.ncol : 65536);
+  MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigensolver grid too large");
+  const unsigned nb = (unsigned)a.ncol;  // one column per block
   hipLaunchKernelGGL(k_trd_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
                      prior, de, a.U);
   MDA_LAUNCH_CHECK();

@@ -1782,6 +1782,118 @@ void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol, int64_t nlev,
   xm_apply<K>(X, ncol, nlev, col0 + g0, ncl, store, tb, xs, status);
 }
 
+// 4''''. X_a = X T' for K = 128 on FP64 MFMA: a block of 8 wavefronts takes
+// X2_C = 2 consecutive columns, four wavefronts per column, each holding T'
+// for two of the eight 16-member output tiles in registers (B layout,
+// tb[s][tt] = T'[4s + kk][16 (2 tp + tt) + i16], 64 doubles per lane).  X is
+// staged through LDS as [column][member][16 levels] from 16-byte column
+// segments, X_a leaves the same way.  `only` as in the 4-column kernel.
+constexpr int X2_C = 2;
+
+template <int K>
+__global__ __launch_bounds__(512) void k_letkf_transform_mfma2(
+    double* __restrict__ X, int64_t ncol, int64_t nlev, const double* __restrict__ T,
+    int64_t col0, int64_t nc, int* __restrict__ status, const int* __restrict__ only) {
+  static_assert(K == 128, "eight 16-member tiles over four wavefronts");
+  if (*status & MDA_ST_SKIP_UPDATE) return;
+  constexpr int NS = K / 4, NT = 2, NV = X2_C * XM_L * K / 512;
+  __shared__ double xs[X2_C * K * XM_L];  // [column][member or j][level]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c = wv >> 2, tp = wv & 3;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const int64_t n = ncol * nlev;
+  const int64_t nb = gridDim.x;
+  const int64_t vb = (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  const int64_t g0 = vb * X2_C;  // within the pass [col0, col0 + nc)
+  if (g0 >= nc) return;
+  const int ncl = (int)(nc - g0 < X2_C ? nc - g0 : X2_C);
+  unsigned store = (1u << ncl) - 1u;
+  if (only) {
+    unsigned m = 0u;
+    for (int cc = 0; cc < ncl; ++cc) m |= only[g0 + cc] != 0 ? 1u << cc : 0u;
+    store = m;
+    if (store == 0u) return;  // block-uniform
+  }
+  const bool own = (store >> c) & 1u;
+  double tb[NS][NT];
+  {
+    const double* Tg = T + (size_t)(g0 + (own ? c : 0)) * K * K;
+    double cs[NT] = {0.0, 0.0};
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4)
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        tb[s4][tt] = own ? Tg[(size_t)(4 * s4 + kk) * K + 16 * (2 * tp + tt) + i16] : 0.0;
+        cs[tt] += tb[s4][tt];
+      }
+    const double rk = 1.0 / (double)K;
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+      cs[tt] += __shfl_xor(cs[tt], 16);
+      cs[tt] += __shfl_xor(cs[tt], 32);
+      const double dc = 1.0 - cs[tt];
+#pragma unroll
+      for (int s4 = 0; s4 < NS; ++s4) tb[s4][tt] = fma(dc, rk, tb[s4][tt]);
+    }
+  }
+  double pv[NV];
+  auto load_chunk = [&](int64_t l0) {
+    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 512 * q;
+      const int cc = idx % X2_C, l = (idx / X2_C) % XM_L, m = idx / (X2_C * XM_L);
+      double v = 0.0;
+      if (cc < ncl && l < nll) v = X[(size_t)m * n + (l0 + l) * ncol + col0 + g0 + cc];
+      pv[q] = v;
+    }
+  };
+  load_chunk(0);
+  for (int64_t l0 = 0; l0 < nlev; l0 += XM_L) {
+    const int nll = (int)(nlev - l0 < XM_L ? nlev - l0 : XM_L);
+    __syncthreads();  // the previous chunk's output has left xs
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 512 * q;
+      const int cc = idx % X2_C, l = (idx / X2_C) % XM_L, m = idx / (X2_C * XM_L);
+      xs[(cc * K + m) * XM_L + l] = pv[q];
+    }
+    __syncthreads();
+    if (l0 + XM_L < nlev) load_chunk(l0 + XM_L);
+    d4 acc[NT];
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) acc[tt] = d4{0.0, 0.0, 0.0, 0.0};
+    const double* xa = xs + (c * K + kk) * XM_L + i16;  // A[i16][kk] of step 0
+#pragma unroll
+    for (int s4 = 0; s4 < NS; ++s4) {
+      const double av = xa[4 * s4 * XM_L];
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt)
+        acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tb[s4][tt], acc[tt], 0, 0, 0);
+    }
+    __syncthreads();  // every wavefront has read its X
+    if (own) {
+      bool fin = true;
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) fin = fin && isfinite(acc[tt][r]);
+      if (__ballot(!fin) != 0ull && lane == 0) atomicOr(status, MDA_ST_NONFINITE_STATE);
+    }
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        xs[(c * K + 16 * (2 * tp + tt) + i16) * XM_L + kk + 4 * r] = acc[tt][r];
+    __syncthreads();
+    for (int idx = tid; idx < X2_C * XM_L * K; idx += 512) {
+      const int cc = idx % X2_C, l = (idx / X2_C) % XM_L, j = idx / (X2_C * XM_L);
+      if (cc < ncl && l < nll && ((store >> cc) & 1u))
+        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + cc] = xs[(cc * K + j) * XM_L + l];
+    }
+  }
+}
+
 // 4'''. X_a = xbar + X' T with split-precision products, the transform of
 // the BF16x3 form: X' = X - xbar in FP64 (xbar over the members per level),
 // X' = hi + lo and T = hi + lo in bf16 (round to nearest even; lo from the
@@ -1937,8 +2049,16 @@ void launch_letkf_transform(hipStream_t s, double* X, int k, int64_t ncol, int64
   if (nc < 0) nc = ncol;
   MDA_REQUIRE(col0 >= 0 && col0 + nc <= ncol, "transform column range out of bounds");
   if (nc == 0) return;
-  MDA_REQUIRE(!only || (precision != MDA_PREC_BF16X3 && k % 16 == 0 && k <= 64),
+  MDA_REQUIRE(!only || (precision != MDA_PREC_BF16X3 && ((k % 16 == 0 && k <= 64) || k == 128)),
               "a column-restricted transform needs the FP64 MFMA path");
+  if (k == 128 && precision != MDA_PREC_BF16X3) {
+    MDA_REQUIRE((nc + X2_C - 1) / X2_C < (int64_t)1 << 31, "transform grid too large");
+    const unsigned nb2 = (unsigned)(((nc + X2_C - 1) / X2_C + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_letkf_transform_mfma2<128>, dim3(nb2), dim3(512), 0, s, X, ncol, nlev, T,
+                       col0, nc, status, only);
+    MDA_LAUNCH_CHECK();
+    return;
+  }
   if (precision == MDA_PREC_BF16X3 && k % 16 == 0 && k <= 64) {
     MDA_REQUIRE((nc + XM_C - 1) / XM_C < (int64_t)1 << 31, "transform grid too large");
     const unsigned nbm = (unsigned)(((nc + XM_C - 1) / XM_C + 7) / 8 * 8);
