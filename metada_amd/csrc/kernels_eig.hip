@@ -1039,16 +1039,17 @@ __device__ inline double pair_swap(double v) { return dpp_f64<0xB1>(v); }
 // the thread pair (2j, 2j + 1) of one wavefront; thread 2j + h holds the
 // components i = 2m + h (m < N/2) in registers, so the pair shares every dot
 // product and update evenly, nothing spills, and the block keeps all four
-// SIMDs of its CU busy (one column per CU: 155 KB of LDS).
+// SIMDs of its CU busy (one column per CU: 147 KB of LDS).
 //   Twisted factorisation with its recurrences in Z (LDS): the even thread
 //   runs the top-down recurrence D+_i, the odd one the bottom-up D-_i, side
 //   by side; each stores the first half of its sequence and, over the
 //   second half, forms gamma_i = D+_i + D-_i - (d_i - l) from the partner's
 //   stored value.  After the twist kt is known, the one missing stretch of
 //   D+ or D- is recomputed, then the vector is built from kt downwards (even
-//   thread) and upwards (odd thread) at once.  Checks, near-pair (Lowdin)
-//   step, back-transform, W on FP64 MFMA and the T layout as in the
-//   one-wavefront kernel.
+//   thread) and upwards (odd thread) at once.  The back-transform runs in
+//   compact-WY blocks of 16 reflectors on FP64 MFMA.  Checks, near-pair
+//   (Lowdin) step, W on FP64 MFMA and the T layout as in the one-wavefront
+//   kernel.
 template <int N>
 __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
@@ -1056,11 +1057,10 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     const double* __restrict__ rho_col, int prior, int combine, double null_rel,
     const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
   static_assert(N == 128, "four wavefronts per column");
-  constexpr int n = N, ld = N + 1, BAND = 8, RC = 8, H = N / 2, M = N / 2;
+  constexpr int n = N, ld = N + 1, BAND = 8, H = N / 2, M = N / 2;
   __shared__ double Z[N * ld];
   __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N], sk[N];
   __shared__ double dots[N][BAND];
-  __shared__ double ur[RC * N];
   __shared__ int s_why, s_near;
   const int t = threadIdx.x, j = t >> 1, h = t & 1;
   const int64_t g = blockIdx.x;
@@ -1243,47 +1243,119 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     if (t == 0) flag[g] = s_why;
     return;  // block-uniform
   }
-  // ---- z <- Q z, Householder rows staged through LDS RC at a time ----
+  // ---- z <- Q z in compact-WY blocks on FP64 MFMA ----
+  // Q = Q_0 Q_1 ... Q_7, Q_b = H_16b ... H_16b+15 = I - V S V^T with V the
+  // block's 16 Householder vectors (rows U[jj][i], i >= 1) and S upper
+  // triangular, S^-1 = diag(1/tau) + strictly_upper(V^T V).  The vectors
+  // move from Z into accumulator tiles (wavefront wv: vectors 32 wv .. 32 wv
+  // + 31, all 128 components: zt[ci][vv][q] = Z[16 ci + kk + 4 q][32 wv +
+  // 16 vv + i16], which is also the B layout of Z as a K = component
+  // operand); per block, last block first:
+  //   W1 = V^T Z (MFMA), S W1 by back-substitution (S^-1 W2 = W1, rows from
+  //   the last reflector up; tau = 0 gives a zero row), Z -= V W2 (MFMA).
+  // V rows at components <= 16 b are zero, so component tiles below b are
+  // skipped.  Z's LDS area is free meanwhile: the V blocks (double
+  // buffered, the next one loaded during this one), V^T V and W live there.
   {
+    const int lane = t & 63, wv = t >> 6;
+    const int i16 = lane & 15, kk = lane >> 4;
+    constexpr int NB = 8, BW = 16;  // blocks of 16 reflectors (126 real ones)
     const double* Ucol = Ug + (size_t)g * n * n;
-    constexpr int NC = (N - 2 + RC - 1) / RC;
+    d4 zt[NB][2];
 #pragma unroll
-    for (int c = NC - 1; c >= 0; --c) {
-      const int jlo = RC * c;
-      const int jhi = (RC * c + RC - 1 < N - 3) ? RC * c + RC - 1 : N - 3;
-      __syncthreads();  // previous chunk's readers are done
-      for (int idx = t; idx < (jhi - jlo + 1) * N; idx += 2 * N)
-        ur[idx] = Ucol[(size_t)jlo * N + idx];
+    for (int ci = 0; ci < NB; ++ci)
+#pragma unroll
+      for (int vv = 0; vv < 2; ++vv)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          zt[ci][vv][q] = Z[(16 * ci + kk + 4 * q) * ld + 32 * wv + 16 * vv + i16];
+    __syncthreads();  // Z's area is reused below
+    double* vbuf0 = Z;               // [16][N] block b even
+    double* vbuf1 = Z + BW * N;      // [16][N] block b odd
+    double* Gs = Z + 2 * BW * N;     // [16][16] V^T V
+    double* Ws = Gs + BW * BW;       // [16][N] W1 -> W2
+    constexpr int NPF = BW * N / (2 * N);  // staged doubles per thread
+    double pf[NPF];
+    auto fetch = [&](int b) {
+#pragma unroll
+      for (int p = 0; p < NPF; ++p) {
+        const int idx = t + 2 * N * p, c = idx / N, i = idx % N, jj = BW * b + c;
+        pf[p] = jj <= N - 3 ? Ucol[(size_t)jj * N + i] : 0.0;
+      }
+    };
+    auto put = [&](double* dst) {
+#pragma unroll
+      for (int p = 0; p < NPF; ++p) dst[t + 2 * N * p] = pf[p];
+    };
+    fetch(NB - 1);
+    put(((NB - 1) & 1) ? vbuf1 : vbuf0);
+#pragma unroll
+    for (int b = NB - 1; b >= 0; --b) {
+      const double* V = (b & 1) ? vbuf1 : vbuf0;  // V[c * N + i] = component i of vector 16b + c
+      __syncthreads();  // block b staged; block b + 1's readers are done
+      if (b > 0) fetch(b - 1);
+      // W1 = V^T Z for the wavefront's two vector tiles; wavefront 0 also V^T V
+      d4 w1[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+      d4 gg = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 4 * b; s < N / 4; ++s) {
+        const int comp = 4 * s + kk;
+        const double av = comp == 0 ? 0.0 : V[i16 * N + comp];  // A = V^T, B = V for V^T V
+#pragma unroll
+        for (int vv = 0; vv < 2; ++vv)
+          w1[vv] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, zt[s / 4][vv][s % 4], w1[vv], 0, 0, 0);
+        if (wv == 0) gg = __builtin_amdgcn_mfma_f64_16x16x4f64(av, av, gg, 0, 0, 0);
+      }
+#pragma unroll
+      for (int vv = 0; vv < 2; ++vv)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ws[(kk + 4 * q) * N + 32 * wv + 16 * vv + i16] = w1[vv][q];
+      if (wv == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Gs[(kk + 4 * q) * BW + i16] = gg[q];
+      }
       __syncthreads();
-      // components i >= jlo + 1 take part: m >= jlo / 2, except i = jlo
-      const int m0 = jlo / 2;
-#pragma unroll 1
-      for (int jj = jhi; jj >= jlo; --jj) {
-        const double* u = ur + (jj - jlo) * N;
-        const double tau = u[0];
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+      // W2 = S W1: W2[m] = tau_m (W1[m] - sum_{c > m} G[m][c] W2[c]), one
+      // vector per thread
+      if (t < N) {
+        double a[BW];
 #pragma unroll
-        for (int m = m0; m < M; ++m) {
-          if ((m & 7) == 0) lds_reload_point();
-          const double uu = (m == m0 && h == 0) ? 0.0 : u[2 * m + h];
-          const double tt = uu * z[m];
-          switch (m & 3) {
-            case 0: p0 += tt; break;
-            case 1: p1 += tt; break;
-            case 2: p2 += tt; break;
-            default: p3 += tt; break;
-          }
-        }
-        double dot = (p0 + p1) + (p2 + p3);
-        dot = tau * (dot + pair_swap(dot));
+        for (int c = 0; c < BW; ++c) a[c] = Ws[c * N + t];
 #pragma unroll
-        for (int m = m0; m < M; ++m) {
-          if ((m & 7) == 0) lds_reload_point();
-          const double uu = (m == m0 && h == 0) ? 0.0 : u[2 * m + h];
-          z[m] -= dot * uu;
+        for (int m = BW - 1; m >= 0; --m) {
+          const double w = V[m * N] * a[m];  // V[m][0] holds tau
+          Ws[m * N + t] = w;
+#pragma unroll
+          for (int c = 0; c < m; ++c) a[c] = fma(-Gs[c * BW + m], w, a[c]);
         }
       }
+      __syncthreads();
+      // Z -= V W2 on the component tiles at or above b
+#pragma unroll
+      for (int ci = b; ci < NB; ++ci) {
+        const int comp = 16 * ci + i16;
+#pragma unroll
+        for (int s4 = 0; s4 < BW / 4; ++s4) {
+          const double av = comp == 0 ? 0.0 : -V[(4 * s4 + kk) * N + comp];
+#pragma unroll
+          for (int vv = 0; vv < 2; ++vv)
+            zt[ci][vv] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                av, Ws[(4 * s4 + kk) * N + 32 * wv + 16 * vv + i16], zt[ci][vv], 0, 0, 0);
+        }
+      }
+      if (b > 0) put(((b - 1) & 1) ? vbuf1 : vbuf0);
     }
+    __syncthreads();  // the last block's readers are done with Z's area
+#pragma unroll
+    for (int ci = 0; ci < NB; ++ci)
+#pragma unroll
+      for (int vv = 0; vv < 2; ++vv)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          Z[(16 * ci + kk + 4 * q) * ld + 32 * wv + 16 * vv + i16] = zt[ci][vv][q];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) z[m] = Zj[(2 * m + h) * ld];
   }
   // ---- U_s = Z diag(f), y' for wbar ----
   const double sc = pri ? 1.0 : r;
