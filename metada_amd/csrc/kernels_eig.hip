@@ -1056,10 +1056,16 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     const double* __restrict__ lam, const double* __restrict__ bvec, double rho,
     const double* __restrict__ rho_col, int prior, int combine, double null_rel,
     const double* __restrict__ Ug, double* __restrict__ T, int* __restrict__ flag) {
+  // N = 64 (two wavefronts, four columns per CU) measured slower than the
+  // one-wavefront kernel at 1024 x 512 x 40: 33.6 against 29.0 ms
   static_assert(N == 128, "four wavefronts per column");
   constexpr int n = N, ld = N + 1, BAND = 8, H = N / 2, M = N / 2;
+  constexpr int NW = 2 * N / 64;  // wavefronts per column
   __shared__ double Z[N * ld];
-  __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N], sk[N];
+  // sk (column sums of U_s, after the back-transform) reuses swt (near-pair
+  // weights)
+  __shared__ double sd[N], se[N], sl[N], sb[N], swt[N], sy[N];
+  double* sk = swt;
   __shared__ double dots[N][BAND];
   __shared__ int s_why, s_near;
   const int t = threadIdx.x, j = t >> 1, h = t & 1;
@@ -1244,11 +1250,11 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     return;  // block-uniform
   }
   // ---- z <- Q z in compact-WY blocks on FP64 MFMA ----
-  // Q = Q_0 Q_1 ... Q_7, Q_b = H_16b ... H_16b+15 = I - V S V^T with V the
+  // Q = Q_0 Q_1 ... Q_{N/16-1}, Q_b = H_16b ... H_16b+15 = I - V S V^T with V the
   // block's 16 Householder vectors (rows U[jj][i], i >= 1) and S upper
   // triangular, S^-1 = diag(1/tau) + strictly_upper(V^T V).  The vectors
   // move from Z into accumulator tiles (wavefront wv: vectors 32 wv .. 32 wv
-  // + 31, all 128 components: zt[ci][vv][q] = Z[16 ci + kk + 4 q][32 wv +
+  // + 31, all N components: zt[ci][vv][q] = Z[16 ci + kk + 4 q][32 wv +
   // 16 vv + i16], which is also the B layout of Z as a K = component
   // operand); per block, last block first:
   //   W1 = V^T Z (MFMA), S W1 by back-substitution (S^-1 W2 = W1, rows from
@@ -1259,7 +1265,7 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
   {
     const int lane = t & 63, wv = t >> 6;
     const int i16 = lane & 15, kk = lane >> 4;
-    constexpr int NB = 8, BW = 16;  // blocks of 16 reflectors (126 real ones)
+    constexpr int NB = N / 16, BW = 16;  // blocks of 16 reflectors (N - 2 real ones)
     const double* Ucol = Ug + (size_t)g * n * n;
     d4 zt[NB][2];
 #pragma unroll
@@ -1411,7 +1417,7 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
   int tile = 0;
   for (int ti = 0; ti < nt; ++ti) {
     for (int tj = 0; tj <= ti; ++tj, ++tile) {
-      if ((tile & 3) != wv) continue;
+      if ((tile % NW) != wv) continue;
       d4 am = {0.0, 0.0, 0.0, 0.0};
       const double* ra = Z + (16 * ti + i16) * ld + kk;
       const double* rb = Z + (16 * tj + i16) * ld + kk;
