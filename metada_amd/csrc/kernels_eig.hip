@@ -857,92 +857,11 @@ __global__ __launch_bounds__(256) void k_trd_vectors_x(
 
 // ---------------------------------------------------------------------------
 // Ensembles wider than one wavefront (k = 128): the same three stages with
-// two wavefronts per column (one row / one eigenvalue per thread) and, for
-// the eigenvalues, bisection on Sturm counts.
+// four wavefronts per column for E1 and E3 and, for the eigenvalues,
+// bisection on Sturm counts.
 // ---------------------------------------------------------------------------
 
-// Sum over the block's two wavefronts; `red` holds one slot per wavefront.
-// Ends with a block barrier, so the caller may read shared values written
-// before the call.
-__device__ inline double block2_total(double v, double* red) {
-  const double s = wave_total(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  return red[0] + red[1];
-}
-
-// E1 for N = 128: thread r keeps row r of A in registers.  Per step three
-// block barriers: |x|^2 with the pivot and diagonal, u, and p^T u.  Steps in
-// chunks of E1W_CH (static column ranges start at the chunk).
-constexpr int E1W_CH = 8;
-template <int N>
-__global__ __launch_bounds__(N) __attribute__((amdgpu_waves_per_eu(1)))
-void k_trd_w(int64_t ncol, const double* __restrict__ C, const int* __restrict__ nloc,
-             double rho, const double* __restrict__ rho_col, int prior,
-             double* __restrict__ de, double* __restrict__ Uout) {
-  static_assert(N == 128, "two wavefronts per column");
-  constexpr int n = N;
-  __shared__ double ub[N], wb[N], red[2][2], xs[2];
-  const int t = threadIdx.x;
-  {  // one column per block
-    const int64_t g = blockIdx.x;
-    if (g >= ncol || nloc[g] == 0) return;
-    const double r = rho_col ? rho_col[g] : rho;
-    const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
-    const double* Cg = C + (size_t)g * n * n;
-    double ar[N];
-#pragma unroll
-    for (int c = 0; c < N; ++c) ar[c] = Cg[(size_t)t * n + c] + (t == c ? sigma : 0.0);
-    double* U = Uout + (size_t)g * n * n;
-    double* dg = de + (size_t)g * 2 * n;
-    double xc = ar[0];  // A[t][j] of the current step
-    constexpr int NCH = (N - 2 + E1W_CH - 1) / E1W_CH;
-#pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) {
-      const int c0 = E1W_CH * ch;
-      const int jend = (E1W_CH * ch + E1W_CH < n - 2) ? E1W_CH * ch + E1W_CH : n - 2;
-#pragma unroll 1
-      for (int j = c0; j < jend; ++j) {
-        const double xi = t > j ? xc : 0.0;
-        if (t == j + 1) xs[0] = xc;
-        if (t == j) xs[1] = xc;
-        const double nrm2 = block2_total(xi * xi, red[0]);
-        const double x0 = xs[0];
-        const bool nz = nrm2 > 0.0;
-        const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
-        const double ui = nz ? xi - (t == j + 1 ? alpha : 0.0) : 0.0;
-        const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
-        if (t == 0) { dg[j] = xs[1]; dg[n + j] = alpha; }
-        U[(size_t)j * n + t] = t == 0 ? tau : ui;
-        ub[t] = ui;
-        __syncthreads();
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-#pragma unroll
-        for (int c = c0; c < N; c += 4) {
-          p0 += ar[c] * ub[c];
-          p1 += ar[c + 1] * ub[c + 1];
-          p2 += ar[c + 2] * ub[c + 2];
-          p3 += ar[c + 3] * ub[c + 3];
-        }
-        const double p = tau * ((p0 + p1) + (p2 + p3));
-        const double K = 0.5 * tau * block2_total(p * ui, red[1]);
-        const double wi = t > j ? p - K * ui : 0.0;
-        wb[t] = wi;
-        __syncthreads();
-#pragma unroll
-        for (int c = c0; c < N; ++c) {
-          if ((c & 7) == 0) lds_reload_point();
-          ar[c] = fma(-ui, wb[c], fma(-wi, ub[c], ar[c]));  // two FMAs
-          if (c > c0 && c <= c0 + E1W_CH && c == j + 1) xc = ar[c];
-        }
-      }
-    }
-    // last 2x2 block straight from the owners' registers
-    if (t == n - 2) { dg[n - 2] = ar[n - 2]; dg[2 * n - 2] = ar[n - 1]; }
-    if (t == n - 1) { dg[n - 1] = ar[n - 1]; dg[2 * n - 1] = 0.0; }
-    __syncthreads();
-  }
-}
+constexpr int E1W_CH = 8;  // step chunk of the k = 128 E1 (static column ranges)
 
 // E2 by bisection on Sturm counts (one wavefront per column, lane j owns
 // eigenvalues j, j + 64, ...): count(x) = #{i : q_i < 0} with
@@ -1034,6 +953,155 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
 // Swaps a double between the two lanes of each even/odd pair (DPP
 // quad_perm [1, 0, 3, 2]).
 __device__ inline double pair_swap(double v) { return dpp_f64<0xB1>(v); }
+
+// Sum over the block's four wavefronts; `red` holds one slot per wavefront.
+// Ends with a block barrier.
+__device__ inline double block4_total(double v, double* red) {
+  const double s = wave_total(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// E1 for N = 128 with four wavefronts per column: R consecutive rows belong
+// to a group of G = 2R threads, thread h of the group holding those rows'
+// entries of the columns G m + h (64 doubles in all).  At 1024 x 512 x 40:
+// R = 1 (thread pairs) 260 ms, R = 2 219 ms, R = 4 239 ms per analysis (more
+// row sums per step, no further triangle savings); R = 2 runs.
+// Every u and w value read from LDS feeds R rows' FMAs, and two columns share
+// a CU at two waves per SIMD.  The same Householder steps as the
+// one-row-per-thread form: |x|^2 and p^T u by block reductions, A u by
+// partial row sums combined across the group by DPP, the rank-2 update on
+// each thread's share of the rows.
+template <int N, int R>
+__global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double* __restrict__ C,
+                                                     const int* __restrict__ nloc, double rho,
+                                                     const double* __restrict__ rho_col,
+                                                     int prior, double* __restrict__ de,
+                                                     double* __restrict__ Uout) {
+  static_assert(N == 128 && (R == 2 || R == 4), "four wavefronts per column");
+  constexpr int n = N, G = 2 * R, M = N / G;
+  static_assert(E1W_CH % G == 0, "chunks start on a column group");
+  __shared__ double ub[N], wb[N], red[2][4], xs[2];
+  const int t = threadIdx.x, h = t & (G - 1);
+  const int rb = (t / G) * R;  // first row of the group
+  const int64_t g = blockIdx.x;
+  if (g >= ncol || nloc[g] == 0) return;  // block-uniform
+  const double rr = rho_col ? rho_col[g] : rho;
+  const double sigma = prior ? (double)(n - 1) / rr : (double)(n - 1);
+  const double* Cg = C + (size_t)g * n * n;
+  double a[R][M];  // a[i][m] = A[rb + i][G m + h]
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int c = G * m + h;
+      a[i][m] = Cg[(size_t)(rb + i) * n + c] + (rb + i == c ? sigma : 0.0);
+    }
+  double* U = Uout + (size_t)g * n * n;
+  double* dg = de + (size_t)g * 2 * n;
+  const int lbase = (t & 63) & ~(G - 1);  // the group's first lane
+  // xc[i]: A[rb + i][j] of the current step, held by the whole group
+  double xc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) xc[i] = __shfl(a[i][0], lbase);
+  // sum over the group's lanes, left in every lane
+  auto group_sum = [&](double v) {
+    v += dpp_f64<0xB1>(v);                   // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);                   // quad_perm [2, 3, 0, 1]
+    if constexpr (G == 8) v += dpp_f64<0x141>(v);  // row_half_mirror
+    return v;
+  };
+  constexpr int NCH = (N - 2 + E1W_CH - 1) / E1W_CH;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int c0 = E1W_CH * ch, m0 = c0 / G;
+    const int jend = (E1W_CH * ch + E1W_CH < n - 2) ? E1W_CH * ch + E1W_CH : n - 2;
+#pragma unroll 1
+    for (int j = c0; j < jend; ++j) {
+      double xi[R], s2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        xi[i] = rb + i > j ? xc[i] : 0.0;
+        s2 = fma(xi[i], xi[i], s2);
+        if (h == 0 && rb + i == j + 1) xs[0] = xc[i];
+        if (h == 0 && rb + i == j) xs[1] = xc[i];
+      }
+      const double nrm2 = block4_total(h == 0 ? s2 : 0.0, red[0]);
+      const double x0 = xs[0];
+      const bool nz = nrm2 > 0.0;
+      const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
+      const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
+      double u[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) u[i] = nz ? xi[i] - (rb + i == j + 1 ? alpha : 0.0) : 0.0;
+      if (t == 0) { dg[j] = xs[1]; dg[n + j] = alpha; }
+      if (h == 0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          U[(size_t)j * n + rb + i] = rb + i == 0 ? tau : u[i];
+          ub[rb + i] = u[i];
+        }
+      }
+      __syncthreads();
+      double p[R], q[R], pu = 0.0;  // two accumulation chains per row
+#pragma unroll
+      for (int i = 0; i < R; ++i) p[i] = q[i] = 0.0;
+#pragma unroll
+      for (int m = m0; m < M; ++m) {
+        const double uc = ub[G * m + h];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          if (m & 1) q[i] = fma(a[i][m], uc, q[i]);
+          else p[i] = fma(a[i][m], uc, p[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        p[i] = tau * group_sum(p[i] + q[i]);
+        pu = fma(p[i], u[i], pu);
+      }
+      const double K = 0.5 * tau * block4_total(h == 0 ? pu : 0.0, red[1]);
+      double w[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) w[i] = rb + i > j ? p[i] - K * u[i] : 0.0;
+      if (h == 0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) wb[rb + i] = w[i];
+      }
+      __syncthreads();
+      double nx[R];  // A[rb + i][j + 1] if this thread holds column j + 1
+#pragma unroll
+      for (int i = 0; i < R; ++i) nx[i] = 0.0;
+#pragma unroll
+      for (int m = m0; m < M; ++m) {
+        if ((m & 3) == 0) lds_reload_point();
+        const int c = G * m + h;
+        const double wc = wb[c], uc = ub[c];
+        const bool take = m <= m0 + E1W_CH / G && c == j + 1;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          a[i][m] = fma(-u[i], wc, fma(-w[i], uc, a[i][m]));  // two FMAs per entry
+          if (take) nx[i] = a[i][m];
+        }
+      }
+      const int src = lbase + ((j + 1) & (G - 1));
+#pragma unroll
+      for (int i = 0; i < R; ++i) xc[i] = __shfl(nx[i], src);
+    }
+  }
+  // last 2x2 block from the owners' registers (row n-2 gives e_{n-2}):
+  // rows n-2, n-1 are the last two rows of the last group, columns n-2 and
+  // n-1 the lanes G-2 and G-1 at m = M-1
+  if (rb + R - 2 == n - 2) {
+    if (h == G - 2) dg[n - 2] = a[R - 2][M - 1];
+    if (h == G - 1) {
+      dg[2 * n - 2] = a[R - 2][M - 1];
+      dg[n - 1] = a[R - 1][M - 1];
+      dg[2 * n - 1] = 0.0;
+    }
+  }
+}
 
 // E3 for N = 128 with four wavefronts per column.  Eigenvector j belongs to
 // the thread pair (2j, 2j + 1) of one wavefront; thread 2j + h holds the
@@ -1483,8 +1551,8 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
   launch_zero(s, flag, sizeof(int) * a.ncol);
   MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigensolver grid too large");
   const unsigned nb = (unsigned)a.ncol;  // one column per block
-  hipLaunchKernelGGL(k_trd_w<N>, dim3(nb), dim3(N), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
-                     prior, de, a.U);
+  hipLaunchKernelGGL((k_trd_wr<N, 2>), dim3(nb), dim3(2 * N), 0, s, a.ncol, a.C, a.nloc, a.rho,
+                     a.rho_col, prior, de, a.U);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
   hipLaunchKernelGGL(k_trd_eigvals_bisect<N>, dim3((unsigned)((a.ncol + 3) / 4)), dim3(256), 0,
