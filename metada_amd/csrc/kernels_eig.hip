@@ -954,20 +954,24 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
 // quad_perm [1, 0, 3, 2]).
 __device__ inline double pair_swap(double v) { return dpp_f64<0xB1>(v); }
 
-// Sum over the block's four wavefronts; `red` holds one slot per wavefront.
+// Sum over the block's NW wavefronts; `red` holds one slot per wavefront.
 // Ends with a block barrier.
-__device__ inline double block4_total(double v, double* red) {
+template <int NW>
+__device__ inline double block_total(double v, double* red) {
   const double s = wave_total(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  return (red[0] + red[1]) + (red[2] + red[3]);
+  if constexpr (NW == 4) return (red[0] + red[1]) + (red[2] + red[3]);
+  else return red[0] + red[1];
 }
 
 // E1 for N = 128 with four wavefronts per column: R consecutive rows belong
 // to a group of G = 2R threads, thread h of the group holding those rows'
 // entries of the columns G m + h (64 doubles in all).  At 1024 x 512 x 40:
 // R = 1 (thread pairs) 260 ms, R = 2 219 ms, R = 4 239 ms per analysis (more
-// row sums per step, no further triangle savings); R = 2 runs.
+// row sums per step, no further triangle savings); R = 2 runs.  At N = 64
+// (two wavefronts, four waves per SIMD) it took 26.7 ms against 19.8 ms for
+// the one-wavefront k_trd, whose steps need no block barrier.
 // Every u and w value read from LDS feeds R rows' FMAs, and two columns share
 // a CU at two waves per SIMD.  The same Householder steps as the
 // one-row-per-thread form: |x|^2 and p^T u by block reductions, A u by
@@ -980,7 +984,7 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
                                                      int prior, double* __restrict__ de,
                                                      double* __restrict__ Uout) {
   static_assert(N == 128 && (R == 2 || R == 4), "four wavefronts per column");
-  constexpr int n = N, G = 2 * R, M = N / G;
+  constexpr int n = N, G = 2 * R, M = N / G, NW = 2 * N / 64;
   static_assert(E1W_CH % G == 0, "chunks start on a column group");
   __shared__ double ub[N], wb[N], red[2][4], xs[2];
   const int t = threadIdx.x, h = t & (G - 1);
@@ -1027,7 +1031,7 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
         if (h == 0 && rb + i == j + 1) xs[0] = xc[i];
         if (h == 0 && rb + i == j) xs[1] = xc[i];
       }
-      const double nrm2 = block4_total(h == 0 ? s2 : 0.0, red[0]);
+      const double nrm2 = block_total<NW>(h == 0 ? s2 : 0.0, red[0]);
       const double x0 = xs[0];
       const bool nz = nrm2 > 0.0;
       const double alpha = nz ? (x0 > 0.0 ? -1.0 : 1.0) * sqrt(nrm2) : 0.0;
@@ -1061,7 +1065,7 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
         p[i] = tau * group_sum(p[i] + q[i]);
         pu = fma(p[i], u[i], pu);
       }
-      const double K = 0.5 * tau * block4_total(h == 0 ? pu : 0.0, red[1]);
+      const double K = 0.5 * tau * block_total<NW>(h == 0 ? pu : 0.0, red[1]);
       double w[R];
 #pragma unroll
       for (int i = 0; i < R; ++i) w[i] = rb + i > j ? p[i] - K * u[i] : 0.0;
