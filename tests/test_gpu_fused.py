@@ -144,3 +144,41 @@ def test_fused_per_column_rho_matches_oracle(mda, ctx, orc):
                        obs.variances, f.inflation, f.localization_radius, rho_col=rho_col)
     Xo, _, _ = orc.letkf_standard(prob, X, H.apply_host(X))
     assert rel_err(res[0], Xo) < TOL
+
+
+@pytest.mark.parametrize("nlev", [1, 16, 48, 53, 64])
+def test_fused_level_chunks(mda, ctx, orc, nlev):
+    """Level counts at and around the 16-level chunks and the three chunks
+    the eigenvector kernel loads ahead (48): the fused path against the
+    stored-T path (bit for bit) and the oracle, on 9 x 7 columns (a partial
+    group of four)."""
+    case = latlon_case(nlon=9, nlat=7, nlev=nlev, k=32, p=400, seed=300 + nlev, nmodes=20)
+    Xf, ff, _ = _run_latlon(mda, ctx, case, 3000.0, 1.05, keep=False)
+    Xk, _, _ = _run_latlon(mda, ctx, case, 3000.0, 1.05, keep=True)
+    np.testing.assert_array_equal(Xf, Xk)
+    prob = orc.Problem(32, case["nlon"], case["nlat"], nlev, orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], ff.inflation, ff.localization_radius)
+    Xo, _, _ = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert rel_err(Xf, Xo) < TOL
+
+
+@pytest.mark.parametrize("nlev", [3, 20])
+def test_k128_transform_odd_columns(mda, ctx, orc, nlev, monkeypatch):
+    """k = 128 on 15 x 7 columns (the last two-column block holds one) with
+    one and two 16-level chunks: the two-column MFMA transform against the
+    oracle, in one pass and in passes of 40 columns (pass offsets, a short
+    last pass), bit for bit between the two."""
+    case = latlon_case(nlon=15, nlat=7, nlev=nlev, k=128, p=2500, seed=77 + nlev, nmodes=60)
+    monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+    Xg, f, d = _run_latlon(mda, ctx, case, 4000.0, 1.05, keep=False)
+    monkeypatch.setenv("MDA_LETKF_CHUNK_COLS", "40")
+    Xp, _, _ = _run_latlon(mda, ctx, case, 4000.0, 1.05, keep=False)
+    monkeypatch.delenv("MDA_LETKF_CHUNK_COLS", raising=False)
+    np.testing.assert_array_equal(Xg, Xp)
+    prob = orc.Problem(128, case["nlon"], case["nlat"], nlev, orc.COORD_GEOGRAPHIC,
+                       case["col_lat"], case["col_lon"], case["lat"], case["lon"], case["yo"],
+                       case["var"], f.inflation, f.localization_radius)
+    Xo, _, nloc = orc.letkf_standard(prob, case["X"], case["H"].apply_host(case["X"]))
+    assert d["total_local_obs"] == nloc.sum()
+    assert rel_err(Xg, Xo) < TOL
