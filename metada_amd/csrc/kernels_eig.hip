@@ -866,9 +866,9 @@ constexpr int E1W_CH = 8;  // step chunk of the k = 128 E1 (static column ranges
 // E2 by bisection on Sturm counts (one wavefront per column, lane j owns
 // eigenvalues j, j + 64, ...): count(x) = #{i : q_i < 0} with
 // q_0 = d_0 - x, q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}, |q_i| kept >= pivmin
-// (Golub & Van Loan, sec. 8.4.1).  Starts from the Gershgorin interval and
-// halves until the interval is within 2 eps of the eigenvalue.  Ascending
-// by construction.
+// (Golub & Van Loan, sec. 8.4.1).  Starts from the Gershgorin interval cut
+// into 65 cells by one multisection sweep and halves until the interval is
+// within 1e-13 of the eigenvalue (relative).  Ascending by construction.
 template <int N>
 __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
                                                             const int* __restrict__ nloc,
@@ -876,7 +876,8 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
                                                             double* __restrict__ lam,
                                                             int* __restrict__ flag) {
   constexpr int n = N, NQ = (N + 63) / 64;
-  __shared__ double sd[4][N], se2[4][N];
+  __shared__ double sd[4][N], se2[4][N], msx[4][64];
+  __shared__ int msc[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t g = (int64_t)blockIdx.x * 4 + wv;
   if (g >= ncol || nloc[g] == 0) return;  // whole wavefront; no block barriers below
@@ -901,9 +902,42 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
   lo -= 2.0 * eps * tnorm + pivmin;
   hi += 2.0 * eps * tnorm + pivmin;
+  // one multisection sweep: lane L counts the eigenvalues below the L-th of
+  // 64 interior points, and every eigenvalue starts from the cell between
+  // the points whose counts bracket it (six halvings for one sweep)
+  {
+    const double xl = lo + (hi - lo) * ((double)(lane + 1) / 65.0);
+    double qv = 1.0;
+    int cnt = 0;
+    for (int i = 0; i < n; ++i) {
+      const double ei = i > 0 ? e2[i - 1] : 0.0;
+      double rq = __builtin_amdgcn_rcp(qv);
+      rq = fma(rq, fma(-qv, rq, 1.0), rq);
+      double v = (d[i] - xl) - ei * rq;
+      if (fabs(v) < pivmin) v = -pivmin;
+      qv = v;
+      cnt += v < 0.0;
+    }
+    msx[wv][lane] = xl;
+    msc[wv][lane] = cnt;
+  }
+  wave_lds_sync();
   double a[NQ], b[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) { a[q] = lo; b[q] = hi; }
+  for (int q = 0; q < NQ; ++q) {
+    const int j = lane + 64 * q;
+    a[q] = lo;
+    b[q] = hi;
+    for (int m = 0; m < 64; ++m) {
+      const double xm = msx[wv][m];
+      if (msc[wv][m] <= j) a[q] = fmax(a[q], xm);
+      else b[q] = fmin(b[q], xm);
+    }
+  }
+  // converged at a relative width of 1e-13: the eigenvector checks ask for a
+  // residual within 1e-12 of ||T|| and the transform for 1e-10, and the last
+  // eight halvings to 2 eps would add 14 % to the sweeps
+  const double rel = 1e-13;
   for (int it = 0; it < 128; ++it) {
     bool open = false;
     double x[NQ], qv[NQ];
@@ -911,7 +945,7 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       x[q] = 0.5 * (a[q] + b[q]);
-      open |= (lane + 64 * q < n) && b[q] - a[q] > 2.0 * eps * fmax(fabs(a[q]), fabs(b[q])) + pivmin
+      open |= (lane + 64 * q < n) && b[q] - a[q] > rel * fmax(fabs(a[q]), fabs(b[q])) + pivmin
               && x[q] != a[q] && x[q] != b[q];
       qv[q] = 1.0;
       cnt[q] = 0;
