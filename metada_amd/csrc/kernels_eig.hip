@@ -1230,35 +1230,55 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
   // the missing stretch: D-_i for kt < i < H (bottom-up from D-_H), or D+_i
   // for H <= i < kt (top-down from D+_{H-1}), by the even thread, over slots
   // whose stored values are no longer needed
+  // Both loops below run in unrolled blocks of 8 steps with one wave-wide
+  // exit test per block, so the LDS loads (and, for the vector, the
+  // multipliers, which do not depend on the chain) are issued ahead.
   {
     const bool up = kt < H;
     double xq = Zj[(up ? H : H - 1) * ld];
-    for (int r2 = 0; r2 < H; ++r2) {
-      const int i = up ? H - 1 - r2 : H + r2;
-      const bool act = h == 0 && (up ? i >= kt + 1 : i <= kt - 1);
-      if (!wave_any(act)) break;
-      if (act) {
-        const double ee = up ? se[i] : se[i > 0 ? i - 1 : 0];
-        const double x = (sd[i] - l) - ee * fast_div(ee, safe_pivot(xq, tiny));
-        Zj[i * ld] = x;
-        xq = x;
+    for (int r0 = 0; r0 < H; r0 += 8) {
+      if (!wave_any(h == 0 && (up ? H - 1 - r0 >= kt + 1 : H + r0 <= kt - 1))) break;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = up ? H - 1 - (r0 + u) : H + r0 + u;
+        const bool act = h == 0 && (up ? i >= kt + 1 : i <= kt - 1);
+        const int ic = i < 1 ? 1 : (i > N - 1 ? N - 1 : i);  // in bounds when inactive
+        const double ee = up ? se[ic] : se[ic - 1];
+        const double dl = sd[ic] - l;
+        if (act) {
+          const double x = dl - ee * fast_div(ee, safe_pivot(xq, tiny));
+          Zj[i * ld] = x;
+          xq = x;
+        }
       }
     }
   }
   wave_lds_sync();
   // z_kt = 1; even thread downwards z_i = -(e_i / safe(D+_i)) z_{i+1},
-  // odd thread upwards z_{i+1} = -(e_i / safe(D-_{i+1})) z_i
+  // odd thread upwards z_{i+1} = -(e_i / safe(D-_{i+1})) z_i.  Each row is
+  // read (its D value) before it is overwritten (its z), once.
   if (h == 0) Zj[kt * ld] = 1.0;
   {
     double zc = 1.0;
-    for (int r2 = 0; r2 < N - 1; ++r2) {
-      const int i = h ? kt + r2 : kt - 1 - r2;
-      const bool act = h ? i <= N - 2 : i >= 0;
-      if (!wave_any(act)) break;
-      if (act) {
-        const int row = h ? i + 1 : i;
-        zc = -fast_div(se[i], safe_pivot(Zj[row * ld], tiny)) * zc;
-        Zj[row * ld] = zc;
+    for (int r0 = 0; r0 < N - 1; r0 += 8) {
+      if (!wave_any(h ? kt + r0 <= N - 2 : kt - 1 - r0 >= 0)) break;
+      double mlt[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = h ? kt + r0 + u : kt - 1 - r0 - u;
+        const bool act = h ? i <= N - 2 : i >= 0;
+        const int ic = act ? i : 0, rc = act ? (h ? i + 1 : i) : 0;
+        const double m = -fast_div(se[ic], safe_pivot(Zj[rc * ld], tiny));
+        mlt[u] = act ? m : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = h ? kt + r0 + u : kt - 1 - r0 - u;
+        const bool act = h ? i <= N - 2 : i >= 0;
+        if (act) {
+          zc = mlt[u] * zc;
+          Zj[(h ? i + 1 : i) * ld] = zc;
+        }
       }
     }
   }
