@@ -800,7 +800,7 @@ __global__ __launch_bounds__(64) void k_trd_vectors(
                         T, flag, tb, [] {});
 }
 
-constexpr int E3_PRE = 3;  // 16-level chunks of X loaded during E3's W products
+constexpr int E3_PRE = 2;  // 16-level chunks of X loaded during E3's W products
 
 // E3 with the transform fused: a block of XM_C = 4 wavefronts takes 4
 // consecutive columns of the pass (one each, LDS slices side by side), and

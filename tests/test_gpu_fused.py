@@ -146,10 +146,10 @@ def test_fused_per_column_rho_matches_oracle(mda, ctx, orc):
     assert rel_err(res[0], Xo) < TOL
 
 
-@pytest.mark.parametrize("nlev", [1, 16, 48, 53, 64])
+@pytest.mark.parametrize("nlev", [1, 16, 32, 48, 53, 64])
 def test_fused_level_chunks(mda, ctx, orc, nlev):
-    """Level counts at and around the 16-level chunks and the three chunks
-    the eigenvector kernel loads ahead (48): the fused path against the
+    """Level counts at and around the 16-level chunks and the two chunks the
+    eigenvector kernel loads ahead (32): the fused path against the
     stored-T path (bit for bit) and the oracle, on 9 x 7 columns (a partial
     group of four)."""
     case = latlon_case(nlon=9, nlat=7, nlev=nlev, k=32, p=400, seed=300 + nlev, nmodes=20)
