@@ -819,7 +819,7 @@ __global__ __launch_bounds__(256) void k_trd_vectors_x(
     int64_t x_ncol, int64_t nlev, int64_t col0, int* __restrict__ status) {
   static_assert(N % 16 == 0 && N <= 64, "N must be a multiple of 16, at most 64");
   constexpr int SL = e3_shape<N>::SLICE;
-  static_assert(XM_C * N * XM_L <= XM_C * SL, "X staging exceeds the block's LDS");
+  static_assert(xm_xs_size<N>() <= XM_C * SL, "X staging exceeds the block's LDS");
   __shared__ double lds[XM_C * SL];
   __shared__ int sok[XM_C];
   const int tid = threadIdx.x, wv = tid >> 6;

@@ -1752,7 +1752,7 @@ void k_letkf_transform_mfma(double* __restrict__ X, int64_t ncol, int64_t nlev,
   static_assert(K % 16 == 0 && K <= 64, "K must be a multiple of 16, at most 64");
   if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int NTJ = K / 16, NS = K / 4;
-  __shared__ double xs[XM_C * K * XM_L];  // [column][member or j][level]
+  __shared__ double xs[xm_xs_size<K>()];  // staging of xm_apply
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i16 = lane & 15, kk = lane >> 4;
   const int64_t nb = gridDim.x;
@@ -1797,7 +1797,12 @@ __global__ __launch_bounds__(512) void k_letkf_transform_mfma2(
   static_assert(K == 128, "eight 16-member tiles over four wavefronts");
   if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int NS = K / 4, NT = 2, NV = X2_C * XM_L * K / 512;
-  __shared__ double xs[X2_C * K * XM_L];  // [column][member or j][level]
+  // X in as [column][member][level], X_a out as [column][level][j] with rows
+  // of K + 16 and columns 16 doubles apart (64-bit LDS accesses two lanes per
+  // bank pair in the write-out and the store-side reads)
+  constexpr int LJ = K + 16, CSO = XM_L * LJ + 16;
+  constexpr int XS = X2_C * (K * XM_L > CSO ? K * XM_L : CSO);
+  __shared__ double xs[XS];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c = wv >> 2, tp = wv & 3;
   const int i16 = lane & 15, kk = lane >> 4;
@@ -1884,12 +1889,17 @@ __global__ __launch_bounds__(512) void k_letkf_transform_mfma2(
     for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        xs[(c * K + 16 * (2 * tp + tt) + i16) * XM_L + kk + 4 * r] = acc[tt][r];
+        xs[c * CSO + (kk + 4 * r) * LJ + 16 * (2 * tp + tt) + i16] = acc[tt][r];
     __syncthreads();
-    for (int idx = tid; idx < X2_C * XM_L * K; idx += 512) {
-      const int cc = idx % X2_C, l = (idx / X2_C) % XM_L, j = idx / (X2_C * XM_L);
+    // element q of thread tid: column idx % 2, member 32 (w % 4) + idx / 2 % 32,
+    // level w / 4, with idx = tid + 512 q and w = idx / 64
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 512 * q;
+      const int cc = idx % X2_C, w = idx / 64;
+      const int j = 32 * (w % (K / 32)) + (idx / X2_C) % 32, l = w / (K / 32);
       if (cc < ncl && l < nll && ((store >> cc) & 1u))
-        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + cc] = xs[(cc * K + j) * XM_L + l];
+        X[(size_t)j * n + (l0 + l) * ncol + col0 + g0 + cc] = xs[cc * CSO + l * LJ + j];
     }
   }
 }

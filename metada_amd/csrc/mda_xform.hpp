@@ -10,7 +10,13 @@
 // T'[4s + kk][16 tj + i16], lane = i16 + 16 kk) and runs
 // v_mfma_f64_16x16x4f64 over 16-level chunks.  X is staged through LDS as
 // [column][member][16 levels] from 32-byte column segments (coalesced across
-// the block's columns), and X_a leaves the same way.
+// the block's columns), and X_a leaves through LDS as [column][level][member].
+// Strides in doubles (the LDS serves a 64-bit access from 32 double-wide
+// bank pairs): a column's block starts 16 pairs further on (xm_csi, xm_cso),
+// and an output level row is K + 16 long when K is a multiple of 32 (xm_lj),
+// so the staging writes, the product reads, the accumulator write-out (lanes
+// over 16 members and 4 levels) and the store-side reads (lanes over 4
+// columns and 16 members) each fall on every pair exactly twice.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +28,18 @@
 namespace mda {
 
 constexpr int XM_C = 4, XM_L = 16;
+
+template <int K>
+constexpr int xm_lj() { return K % 32 == 0 ? K + 16 : K; }
+template <int K>
+constexpr int xm_csi() { return XM_L * K + 16; }
+template <int K>
+constexpr int xm_cso() { return XM_L * xm_lj<K>() + 16; }
+// LDS doubles of xm_apply's staging buffer
+template <int K>
+constexpr int xm_xs_size() {
+  return XM_C * (xm_csi<K>() > xm_cso<K>() ? xm_csi<K>() : xm_cso<K>());
+}
 
 typedef double xm_d4 __attribute__((ext_vector_type(4)));
 
@@ -77,7 +95,7 @@ __device__ __forceinline__ void xm_load(double (&v)[xm_nv<K>()], const double* _
 // X_a = X T' for the columns colbase + c, c < ncl, of X (ncol columns of
 // nlev levels, member-major), written for the columns whose bit is set in
 // `store`.  Block of 256 threads (wavefront wv: column wv); xs holds
-// XM_C * K * XM_L doubles of LDS.  Every thread of the block calls this (it
+// xm_xs_size<K>() doubles of LDS.  Every thread of the block calls this (it
 // synchronises the block).  The first NPC chunks come from `pre` (loaded by
 // xm_load earlier, so their latency is hidden behind other work); later
 // chunks are loaded one ahead of their products.  A non-finite X_a sets
@@ -92,6 +110,7 @@ __device__ __forceinline__ void xm_apply(double* __restrict__ X, int64_t ncol, i
   constexpr int NTJ = K / 16, NS = K / 4, NV = xm_nv<K>();
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i16 = lane & 15, kk = lane >> 4;
+  constexpr int LJ = xm_lj<K>(), CSI = xm_csi<K>(), CSO = xm_cso<K>();
   const int64_t n = ncol * nlev;
   double pv[NV];
   // one chunk: src staged through xs (src may be pv: it is read before the
@@ -103,14 +122,14 @@ __device__ __forceinline__ void xm_apply(double* __restrict__ X, int64_t ncol, i
     for (int q = 0; q < NV; ++q) {
       const int idx = tid + 256 * q;
       const int c = idx % XM_C, l = (idx / XM_C) % XM_L, m = idx / (XM_C * XM_L);
-      xs[(c * K + m) * XM_L + l] = src[q];
+      xs[c * CSI + m * XM_L + l] = src[q];
     }
     __syncthreads();
     if (load_next) xm_load<K>(pv, X, ncol, nlev, colbase, ncl, l0 + XM_L);
     xm_d4 acc[NTJ];
 #pragma unroll
     for (int tj = 0; tj < NTJ; ++tj) acc[tj] = xm_d4{0.0, 0.0, 0.0, 0.0};
-    const double* xa = xs + (wv * K + kk) * XM_L + i16;  // A[i16][kk] of step 0
+    const double* xa = xs + wv * CSI + kk * XM_L + i16;  // A[i16][kk] of step 0
 #pragma unroll
     for (int s4 = 0; s4 < NS; ++s4) {
       const double av = xa[4 * s4 * XM_L];
@@ -127,16 +146,21 @@ __device__ __forceinline__ void xm_apply(double* __restrict__ X, int64_t ncol, i
         for (int r = 0; r < 4; ++r) fin = fin && isfinite(acc[tj][r]);
       if (__ballot(!fin) != 0ull && lane == 0) atomicOr(status, MDA_ST_NONFINITE_STATE);
     }
-    // D[kk + 4r][i16] of tile tj = X_a[level kk + 4r][j = 16 tj + i16] -> xs[c][j][l]
+    // D[kk + 4r][i16] of tile tj = X_a[level kk + 4r][j = 16 tj + i16] -> xs[c][l][j]
 #pragma unroll
     for (int tj = 0; tj < NTJ; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) xs[(wv * K + 16 * tj + i16) * XM_L + kk + 4 * r] = acc[tj][r];
+      for (int r = 0; r < 4; ++r) xs[wv * CSO + (kk + 4 * r) * LJ + 16 * tj + i16] = acc[tj][r];
     __syncthreads();
-    for (int idx = tid; idx < XM_C * XM_L * K; idx += 256) {
-      const int c = idx % XM_C, l = (idx / XM_C) % XM_L, j = idx / (XM_C * XM_L);
+    // element q of thread tid: column idx % 4, member 16 (w % (K/16)) + idx / 4 % 16,
+    // level w / (K/16), with idx = tid + 256 q and w = idx / 64
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int idx = tid + 256 * q;
+      const int c = idx % XM_C, w = idx / 64;
+      const int j = 16 * (w % (K / 16)) + (idx / XM_C) % 16, l = w / (K / 16);
       if (c < ncl && l < nll && ((store >> c) & 1u))
-        X[(size_t)j * n + (l0 + l) * ncol + colbase + c] = xs[(c * K + j) * XM_L + l];
+        X[(size_t)j * n + (l0 + l) * ncol + colbase + c] = xs[c * CSO + l * LJ + j];
     }
   };
 #pragma unroll
