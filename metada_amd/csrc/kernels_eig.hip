@@ -1169,7 +1169,11 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
   // N = 64 (two wavefronts, four columns per CU) measured slower than the
   // one-wavefront kernel at 1024 x 512 x 40: 33.6 against 29.0 ms
   static_assert(N == 128, "four wavefronts per column");
-  constexpr int n = N, ld = N + 1, BAND = 8, H = N / 2, M = N / 2;
+  // Z's row stride N + 6: 64-bit LDS accesses of an MFMA operand tile (16
+  // rows x 4 columns) fall two lanes per bank pair, of an accumulator tile
+  // (4 rows x 16 columns) at most three (N + 1: four and four), and the
+  // vector-per-thread-pair rows and columns two
+  constexpr int n = N, ld = N + 6, BAND = 8, H = N / 2, M = N / 2;
   constexpr int NW = 2 * N / 64;  // wavefronts per column
   __shared__ double Z[N * ld];
   // sk (column sums of U_s, after the back-transform) reuses swt (near-pair
@@ -1406,10 +1410,15 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
         for (int q = 0; q < 4; ++q)
           zt[ci][vv][q] = Z[(16 * ci + kk + 4 * q) * ld + 32 * wv + 16 * vv + i16];
     __syncthreads();  // Z's area is reused below
-    double* vbuf0 = Z;               // [16][N] block b even
-    double* vbuf1 = Z + BW * N;      // [16][N] block b odd
-    double* Gs = Z + 2 * BW * N;     // [16][16] V^T V
-    double* Ws = Gs + BW * BW;       // [16][N] W1 -> W2
+    // rows of N + 6 (V, read both as the A operand V^T and as the B-side
+    // operand of V W2: two and three lanes per bank pair; rows of N: sixteen
+    // and four) and N + 16 (W, tiles written and read 16 columns wide)
+    constexpr int LV = N + 6, LW = N + 16;
+    static_assert(2 * BW * LV + BW * BW + BW * LW <= N * ld, "V, G and W exceed Z's area");
+    double* vbuf0 = Z;               // [16][LV] block b even
+    double* vbuf1 = Z + BW * LV;     // [16][LV] block b odd
+    double* Gs = Z + 2 * BW * LV;    // [16][16] V^T V
+    double* Ws = Gs + BW * BW;       // [16][LW] W1 -> W2
     constexpr int NPF = BW * N / (2 * N);  // staged doubles per thread
     double pf[NPF];
     auto fetch = [&](int b) {
@@ -1421,13 +1430,16 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
     };
     auto put = [&](double* dst) {
 #pragma unroll
-      for (int p = 0; p < NPF; ++p) dst[t + 2 * N * p] = pf[p];
+      for (int p = 0; p < NPF; ++p) {
+        const int idx = t + 2 * N * p;
+        dst[(idx / N) * LV + idx % N] = pf[p];
+      }
     };
     fetch(NB - 1);
     put(((NB - 1) & 1) ? vbuf1 : vbuf0);
 #pragma unroll
     for (int b = NB - 1; b >= 0; --b) {
-      const double* V = (b & 1) ? vbuf1 : vbuf0;  // V[c * N + i] = component i of vector 16b + c
+      const double* V = (b & 1) ? vbuf1 : vbuf0;  // V[c * LV + i] = component i of vector 16b + c
       __syncthreads();  // block b staged; block b + 1's readers are done
       if (b > 0) fetch(b - 1);
       // W1 = V^T Z for the wavefront's two vector tiles; wavefront 0 also V^T V
@@ -1436,7 +1448,7 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
 #pragma unroll
       for (int s = 4 * b; s < N / 4; ++s) {
         const int comp = 4 * s + kk;
-        const double av = comp == 0 ? 0.0 : V[i16 * N + comp];  // A = V^T, B = V for V^T V
+        const double av = comp == 0 ? 0.0 : V[i16 * LV + comp];  // A = V^T, B = V for V^T V
 #pragma unroll
         for (int vv = 0; vv < 2; ++vv)
           w1[vv] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, zt[s / 4][vv][s % 4], w1[vv], 0, 0, 0);
@@ -1445,7 +1457,7 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
 #pragma unroll
       for (int vv = 0; vv < 2; ++vv)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Ws[(kk + 4 * q) * N + 32 * wv + 16 * vv + i16] = w1[vv][q];
+        for (int q = 0; q < 4; ++q) Ws[(kk + 4 * q) * LW + 32 * wv + 16 * vv + i16] = w1[vv][q];
       if (wv == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) Gs[(kk + 4 * q) * BW + i16] = gg[q];
@@ -1456,11 +1468,11 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
       if (t < N) {
         double a[BW];
 #pragma unroll
-        for (int c = 0; c < BW; ++c) a[c] = Ws[c * N + t];
+        for (int c = 0; c < BW; ++c) a[c] = Ws[c * LW + t];
 #pragma unroll
         for (int m = BW - 1; m >= 0; --m) {
-          const double w = V[m * N] * a[m];  // V[m][0] holds tau
-          Ws[m * N + t] = w;
+          const double w = V[m * LV] * a[m];  // V[m][0] holds tau
+          Ws[m * LW + t] = w;
 #pragma unroll
           for (int c = 0; c < m; ++c) a[c] = fma(-Gs[c * BW + m], w, a[c]);
         }
@@ -1472,11 +1484,11 @@ __global__ __launch_bounds__(2 * N) void k_trd_vectors_w4(
         const int comp = 16 * ci + i16;
 #pragma unroll
         for (int s4 = 0; s4 < BW / 4; ++s4) {
-          const double av = comp == 0 ? 0.0 : -V[(4 * s4 + kk) * N + comp];
+          const double av = comp == 0 ? 0.0 : -V[(4 * s4 + kk) * LV + comp];
 #pragma unroll
           for (int vv = 0; vv < 2; ++vv)
             zt[ci][vv] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                av, Ws[(4 * s4 + kk) * N + 32 * wv + 16 * vv + i16], zt[ci][vv], 0, 0, 0);
+                av, Ws[(4 * s4 + kk) * LW + 32 * wv + 16 * vv + i16], zt[ci][vv], 0, 0, 0);
         }
       }
       if (b > 0) put(((b - 1) & 1) ? vbuf1 : vbuf0);

@@ -1798,10 +1798,12 @@ __global__ __launch_bounds__(512) void k_letkf_transform_mfma2(
   if (*status & MDA_ST_SKIP_UPDATE) return;
   constexpr int NS = K / 4, NT = 2, NV = X2_C * XM_L * K / 512;
   // X in as [column][member][level], X_a out as [column][level][j] with rows
-  // of K + 16 and columns 16 doubles apart (64-bit LDS accesses two lanes per
-  // bank pair in the write-out and the store-side reads)
-  constexpr int LJ = K + 16, CSO = XM_L * LJ + 16;
-  constexpr int XS = X2_C * (K * XM_L > CSO ? K * XM_L : CSO);
+  // of K + 16; each column's block starts 16 bank pairs further on, so the 32
+  // lanes of each half wavefront fall on 32 different 64-bit bank pairs in
+  // the staging writes, the product reads, the write-out and the store-side
+  // reads
+  constexpr int LJ = K + 16, CSI = K * XM_L + 16, CSO = XM_L * LJ + 16;
+  constexpr int XS = X2_C * (CSI > CSO ? CSI : CSO);
   __shared__ double xs[XS];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c = wv >> 2, tp = wv & 3;
@@ -1861,14 +1863,14 @@ __global__ __launch_bounds__(512) void k_letkf_transform_mfma2(
     for (int q = 0; q < NV; ++q) {
       const int idx = tid + 512 * q;
       const int cc = idx % X2_C, l = (idx / X2_C) % XM_L, m = idx / (X2_C * XM_L);
-      xs[(cc * K + m) * XM_L + l] = pv[q];
+      xs[cc * CSI + m * XM_L + l] = pv[q];
     }
     __syncthreads();
     if (l0 + XM_L < nlev) load_chunk(l0 + XM_L);
     d4 acc[NT];
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt) acc[tt] = d4{0.0, 0.0, 0.0, 0.0};
-    const double* xa = xs + (c * K + kk) * XM_L + i16;  // A[i16][kk] of step 0
+    const double* xa = xs + c * CSI + kk * XM_L + i16;  // A[i16][kk] of step 0
 #pragma unroll
     for (int s4 = 0; s4 < NS; ++s4) {
       const double av = xa[4 * s4 * XM_L];

@@ -12,11 +12,13 @@
 // [column][member][16 levels] from 32-byte column segments (coalesced across
 // the block's columns), and X_a leaves through LDS as [column][level][member].
 // Strides in doubles (the LDS serves a 64-bit access from 32 double-wide
-// bank pairs): a column's block starts 16 pairs further on (xm_csi, xm_cso),
-// and an output level row is K + 16 long when K is a multiple of 32 (xm_lj),
-// so the staging writes, the product reads, the accumulator write-out (lanes
-// over 16 members and 4 levels) and the store-side reads (lanes over 4
-// columns and 16 members) each fall on every pair exactly twice.
+// bank pairs, half a wavefront at a time): a column's block starts 8 pairs
+// further on (xm_csi, xm_cso), and an output level row is K + 16 long when K
+// is a multiple of 32 (xm_lj), so in the staging writes (lanes over 4
+// columns and 8 levels per half), the product reads (16 levels x 2
+// members), the accumulator write-out (16 members x 2 levels) and the
+// store-side reads (4 columns x 8 members) the 32 lanes of each half fall on
+// 32 different pairs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,9 +34,9 @@ constexpr int XM_C = 4, XM_L = 16;
 template <int K>
 constexpr int xm_lj() { return K % 32 == 0 ? K + 16 : K; }
 template <int K>
-constexpr int xm_csi() { return XM_L * K + 16; }
+constexpr int xm_csi() { return XM_L * K + 8; }
 template <int K>
-constexpr int xm_cso() { return XM_L * xm_lj<K>() + 16; }
+constexpr int xm_cso() { return XM_L * xm_lj<K>() + 8; }
 // LDS doubles of xm_apply's staging buffer
 template <int K>
 constexpr int xm_xs_size() {
