@@ -142,13 +142,20 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
         // x = 0: tau = u = 0, the step below leaves A unchanged (H = I)
         ub[lane] = ui;
         wave_lds_sync();
+        // the matrix-vector product runs on the rows of the trailing block
+        // only (lanes > j; the rows above are final), so the finished rows
+        // take no share of the LDS broadcast of u (masking the rank-2 update
+        // as well costs 24 spilled registers at three waves per SIMD)
+        const bool trail = lane > j;
         double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        if (trail) {
 #pragma unroll
-        for (int c = c0; c < N; c += 4) {
-          p0 += ar[c] * ub[c];
-          if (c + 1 < N) p1 += ar[c + 1 < N ? c + 1 : c] * ub[c + 1 < N ? c + 1 : c];
-          if (c + 2 < N) p2 += ar[c + 2 < N ? c + 2 : c] * ub[c + 2 < N ? c + 2 : c];
-          if (c + 3 < N) p3 += ar[c + 3 < N ? c + 3 : c] * ub[c + 3 < N ? c + 3 : c];
+          for (int c = c0; c < N; c += 4) {
+            p0 += ar[c] * ub[c];
+            if (c + 1 < N) p1 += ar[c + 1 < N ? c + 1 : c] * ub[c + 1 < N ? c + 1 : c];
+            if (c + 2 < N) p2 += ar[c + 2 < N ? c + 2 : c] * ub[c + 2 < N ? c + 2 : c];
+            if (c + 3 < N) p3 += ar[c + 3 < N ? c + 3 : c] * ub[c + 3 < N ? c + 3 : c];
+          }
         }
         const double p = tau * ((p0 + p1) + (p2 + p3));
         const double K = 0.5 * tau * wave_total(p * ui);
@@ -159,8 +166,10 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
         for (int c = c0; c < N; ++c) {
           if ((c & 7) == 0) lds_reload_point();
           ar[c] = fma(-ui, wb[c], fma(-wi, ub[c], ar[c]));  // two FMAs
-          if (c > c0 && c <= c0 + E1_CH && c == j + 1) xc = ar[c];
         }
+#pragma unroll
+        for (int c = c0 + 1; c <= c0 + E1_CH && c < N; ++c)
+          if (c == j + 1) xc = ar[c];
         wave_lds_sync();
       }
     }
