@@ -1095,16 +1095,22 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
         }
       }
       __syncthreads();
+      // groups whose rows are all final (rb + R - 1 <= j) skip the
+      // matrix-vector product and the update: no share of the LDS
+      // broadcasts of u and w (their diagonal entries were taken before)
+      const bool live = rb + R - 1 > j;
       double p[R], q[R], pu = 0.0;  // two accumulation chains per row
 #pragma unroll
       for (int i = 0; i < R; ++i) p[i] = q[i] = 0.0;
+      if (live) {
 #pragma unroll
-      for (int m = m0; m < M; ++m) {
-        const double uc = ub[G * m + h];
+        for (int m = m0; m < M; ++m) {
+          const double uc = ub[G * m + h];
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-          if (m & 1) q[i] = fma(a[i][m], uc, q[i]);
-          else p[i] = fma(a[i][m], uc, p[i]);
+          for (int i = 0; i < R; ++i) {
+            if (m & 1) q[i] = fma(a[i][m], uc, q[i]);
+            else p[i] = fma(a[i][m], uc, p[i]);
+          }
         }
       }
 #pragma unroll
@@ -1124,16 +1130,18 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
       double nx[R];  // A[rb + i][j + 1] if this thread holds column j + 1
 #pragma unroll
       for (int i = 0; i < R; ++i) nx[i] = 0.0;
+      if (live) {
 #pragma unroll
-      for (int m = m0; m < M; ++m) {
-        if ((m & 3) == 0) lds_reload_point();
-        const int c = G * m + h;
-        const double wc = wb[c], uc = ub[c];
-        const bool take = m <= m0 + E1W_CH / G && c == j + 1;
+        for (int m = m0; m < M; ++m) {
+          if ((m & 3) == 0) lds_reload_point();
+          const int c = G * m + h;
+          const double wc = wb[c], uc = ub[c];
+          const bool take = m <= m0 + E1W_CH / G && c == j + 1;
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-          a[i][m] = fma(-u[i], wc, fma(-w[i], uc, a[i][m]));  // two FMAs per entry
-          if (take) nx[i] = a[i][m];
+          for (int i = 0; i < R; ++i) {
+            a[i][m] = fma(-u[i], wc, fma(-w[i], uc, a[i][m]));  // two FMAs per entry
+            if (take) nx[i] = a[i][m];
+          }
         }
       }
       const int src = lbase + ((j + 1) & (G - 1));
