@@ -849,6 +849,7 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   }
   const int64_t nck = P.chunk;
   ga.C = c->get<double>(pre + "col_C", (size_t)nck * k * k);
+  ga.c32 = prm->precision != MDA_PREC_FP64;
   ga.crec = c->get<int>(pre + "col_rec", (size_t)ncol * LETKF_CREC_INTS);
   ga.b = c->get<double>(pre + "col_bvec", (size_t)ncol * k);
   ga.nloc = c->get<int>(pre + "col_nloc", ncol);
@@ -869,7 +870,7 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   }
 
   letkf_solve_args& sa = P.sa;
-  sa.k = k; sa.ncol = ncol; sa.C = ga.C; sa.b = ga.b; sa.nloc = ga.nloc;
+  sa.k = k; sa.ncol = ncol; sa.C = ga.C; sa.c32 = ga.c32; sa.b = ga.b; sa.nloc = ga.nloc;
   sa.rho = prm->inflation; sa.rho_col = rho_col;
   sa.inflation_form = prm->inflation_form; sa.sqrt_form = prm->sqrt_form;
   sa.combine = prm->combine;

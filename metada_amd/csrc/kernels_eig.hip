@@ -92,10 +92,9 @@ constexpr int E1_CH = 2;  // step chunk of k_trd (static column ranges start at 
 
 template <int N>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-void k_trd(int64_t ncol, const double* __restrict__ C,
-                                            const int* __restrict__ nloc, double rho,
-                                            const double* __restrict__ rho_col, int prior,
-                                            double* __restrict__ de, double* __restrict__ Uout) {
+void k_trd(int64_t ncol, const double* __restrict__ C, int c32,
+           const int* __restrict__ nloc, double rho, const double* __restrict__ rho_col,
+           int prior, double* __restrict__ de, double* __restrict__ Uout) {
   constexpr int n = N;
   __shared__ double row[EIG_N], ub[EIG_N], wb[EIG_N];
   const int lane = threadIdx.x;
@@ -106,14 +105,19 @@ void k_trd(int64_t ncol, const double* __restrict__ C,
     if (g >= ncol || nloc[g] == 0) return;
     const double r = rho_col ? rho_col[g] : rho;
     const double sigma = prior ? (double)(n - 1) / r : (double)(n - 1);
-    const double* Cg = C + (size_t)g * n * n;
     const int rl = lane < n ? lane : n - 1;  // clamped row: loads stay in bounds
     double ar[N];
+    if (c32) {  // FP32 entries, widened
+      const float* Cg = reinterpret_cast<const float*>(C) + (size_t)g * n * n;
 #pragma unroll
-    for (int c = 0; c < N; ++c) {
-      double v = Cg[(size_t)rl * n + c];
-      ar[c] = lane < n ? v + (lane == c ? sigma : 0.0) : 0.0;
+      for (int c = 0; c < N; ++c) ar[c] = (double)Cg[(size_t)rl * n + c];
+    } else {
+      const double* Cg = C + (size_t)g * n * n;
+#pragma unroll
+      for (int c = 0; c < N; ++c) ar[c] = Cg[(size_t)rl * n + c];
     }
+#pragma unroll
+    for (int c = 0; c < N; ++c) ar[c] = lane < n ? ar[c] + (lane == c ? sigma : 0.0) : 0.0;
     double* U = Uout + (size_t)g * n * n;
     double* dg = de + (size_t)g * 2 * n;
     // xc: column j of the current A, lane r holding A[r][j]; taken from the
@@ -1026,7 +1030,8 @@ __device__ inline double block_total(double v, double* red) {
 // each thread's share of the rows.
 template <int N, int R>
 __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double* __restrict__ C,
-                                                     const int* __restrict__ nloc, double rho,
+                                                     int c32, const int* __restrict__ nloc,
+                                                     double rho,
                                                      const double* __restrict__ rho_col,
                                                      int prior, double* __restrict__ de,
                                                      double* __restrict__ Uout) {
@@ -1040,15 +1045,24 @@ __global__ __launch_bounds__(2 * N, 2) void k_trd_wr(int64_t ncol, const double*
   if (g >= ncol || nloc[g] == 0) return;  // block-uniform
   const double rr = rho_col ? rho_col[g] : rho;
   const double sigma = prior ? (double)(n - 1) / rr : (double)(n - 1);
-  const double* Cg = C + (size_t)g * n * n;
   double a[R][M];  // a[i][m] = A[rb + i][G m + h]
+  if (c32) {  // FP32 entries, widened
+    const float* Cg = reinterpret_cast<const float*>(C) + (size_t)g * n * n;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[i][m] = (double)Cg[(size_t)(rb + i) * n + G * m + h];
+  } else {
+    const double* Cg = C + (size_t)g * n * n;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int m = 0; m < M; ++m) a[i][m] = Cg[(size_t)(rb + i) * n + G * m + h];
+  }
 #pragma unroll
   for (int i = 0; i < R; ++i)
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const int c = G * m + h;
-      a[i][m] = Cg[(size_t)(rb + i) * n + c] + (rb + i == c ? sigma : 0.0);
-    }
+    for (int m = 0; m < M; ++m) a[i][m] += rb + i == G * m + h ? sigma : 0.0;
   double* U = Uout + (size_t)g * n * n;
   double* dg = de + (size_t)g * 2 * n;
   const int lbase = (t & 63) & ~(G - 1);  // the group's first lane
@@ -1602,8 +1616,8 @@ static void launch_eig_n(hipStream_t s, const letkf_solve_args& a, double* de, d
   launch_zero(s, flag, sizeof(int) * a.ncol);
   MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigensolver grid too large");
   const unsigned nb = (unsigned)a.ncol;  // one column per block
-  hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.nloc, a.rho, a.rho_col,
-                     prior, de, a.U);
+  hipLaunchKernelGGL(k_trd<N>, dim3(nb), dim3(64), 0, s, a.ncol, a.C, a.c32, a.nloc, a.rho,
+                     a.rho_col, prior, de, a.U);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
   hipLaunchKernelGGL(k_trd_eigvals<N>, dim3((unsigned)((a.ncol + 63) / 64)), dim3(64), 0, s,
@@ -1642,8 +1656,8 @@ static void launch_eig_w(hipStream_t s, const letkf_solve_args& a, double* de, d
   launch_zero(s, flag, sizeof(int) * a.ncol);
   MDA_REQUIRE(a.ncol < (int64_t)1 << 31, "eigensolver grid too large");
   const unsigned nb = (unsigned)a.ncol;  // one column per block
-  hipLaunchKernelGGL((k_trd_wr<N, 2>), dim3(nb), dim3(2 * N), 0, s, a.ncol, a.C, a.nloc, a.rho,
-                     a.rho_col, prior, de, a.U);
+  hipLaunchKernelGGL((k_trd_wr<N, 2>), dim3(nb), dim3(2 * N), 0, s, a.ncol, a.C, a.c32, a.nloc,
+                     a.rho, a.rho_col, prior, de, a.U);
   MDA_LAUNCH_CHECK();
   if (a.ev_stage[0]) MDA_HIP(hipEventRecord(a.ev_stage[0], s));
   hipLaunchKernelGGL(k_trd_eigvals_bisect<N>, dim3((unsigned)((a.ncol + 3) / 4)), dim3(256), 0,

@@ -1126,6 +1126,8 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
         if (16 * t + i16_e < k) a.b[(size_t)g * k + 16 * t + i16_e] = bacc[t] - bsum * rk;
     }
     double* Cg = a.C + (size_t)g * k * k;
+    float* Cf = reinterpret_cast<float*>(a.C) + (size_t)g * k * k;
+    const bool c32 = a.c32 != 0;
     // k through an opaque copy: the element offsets below are recomputed per
     // column instead of being kept live across the candidate loop
     int ko = k;
@@ -1140,8 +1142,13 @@ __device__ __forceinline__ void gram_mixed_body(const letkf_gram_args& a) {
           const int row = 16 * ta + 4 * kq_e + r, col = 16 * tb + i16_e;
           if (row < ko && col < ko) {
             const double v = (double)acc[tile][r] - (rsum[row] + rsum[col]) * rk + tot;
-            Cg[row * ko + col] = v;
-            if (ta != tb) Cg[col * ko + row] = v;
+            if (c32) {
+              Cf[row * ko + col] = (float)v;
+              if (ta != tb) Cf[col * ko + row] = (float)v;
+            } else {
+              Cg[row * ko + col] = v;
+              if (ta != tb) Cg[col * ko + row] = v;
+            }
           }
         }
         ++tile;
@@ -1200,9 +1207,15 @@ __global__ __launch_bounds__(256) void k_letkf_shell_fixup(letkf_gram_args a,
     const double sg = exact_in ? w : -w;
     if (sg == 0.0) continue;
     const double* y = a.yp + (size_t)t * k;
-    double* Cg = a.C + (size_t)g * k * k;
-    for (int idx = lane; idx < k * k; idx += 64)
-      atomicAdd(&Cg[idx], sg * y[idx / k] * y[idx % k]);
+    if (a.c32) {
+      float* Cf = reinterpret_cast<float*>(a.C) + (size_t)g * k * k;
+      for (int idx = lane; idx < k * k; idx += 64)
+        atomicAdd(&Cf[idx], (float)(sg * y[idx / k] * y[idx % k]));
+    } else {
+      double* Cg = a.C + (size_t)g * k * k;
+      for (int idx = lane; idx < k * k; idx += 64)
+        atomicAdd(&Cg[idx], sg * y[idx / k] * y[idx % k]);
+    }
     for (int i = lane; i < k; i += 64) atomicAdd(&a.b[(size_t)g * k + i], sg * y[i] * a.d[t]);
   }
 }
@@ -1347,6 +1360,9 @@ void launch_column_setup(hipStream_t s, const letkf_gram_args& a) {
 
 void launch_letkf_gram(hipStream_t s, const letkf_gram_args& a) {
   MDA_REQUIRE(a.k >= 1 && a.k <= 128, "LETKF Gram supports k <= 128");
+  // FP32 entries of C exactly for the FP32 / BF16x3 contractions (the FP64
+  // kernels write doubles)
+  MDA_REQUIRE((a.c32 != 0) == (a.precision != MDA_PREC_FP64), "Gram storage precision mismatch");
   const int npart = a.k > 64 ? 2 : 1;  // waves per column (FP64 Gram)
   unsigned nb = (unsigned)((a.ncol * (a.precision == MDA_PREC_FP64 ? npart : 1) + 3) / 4);
   if (nb > 262144) nb = 262144;
@@ -1452,10 +1468,12 @@ __global__ __launch_bounds__(256) void k_letkf_solve(letkf_solve_args a, double*
     const double sigma = prior ? (double)(k - 1) / rho : (double)(k - 1);
     const double sc = prior ? 1.0 : rho;
     const double* Cg = a.C + (size_t)g * k * k;
+    const float* Cf = reinterpret_cast<const float*>(a.C) + (size_t)g * k * k;
     for (int idx = tid; idx < kp * kp; idx += nt) {
       int i = idx / kp, j = idx % kp;
       double v;
-      if (i < k && j < k) v = Cg[i * k + j] + (i == j ? sigma : 0.0);
+      if (i < k && j < k)
+        v = (a.c32 ? (double)Cf[i * k + j] : Cg[i * k + j]) + (i == j ? sigma : 0.0);
       else v = (i == j) ? 1.0 : 0.0;  // decoupled pad index (odd k)
       A[i * ld + j] = v;
     }

@@ -256,6 +256,9 @@ struct letkf_gram_args {
   const double* uy;
   const double* uz;
   double* C;    // out [ncol][k][k]
+  // C holds FP32 entries (the FP32 / BF16x3 contractions, whose products are
+  // accumulated in FP32): half the buffer is used, E1 widens them to FP64
+  int c32;
   double* b;    // out [ncol][k]
   int* nloc;    // out [ncol]
   // per-column setup records [ncol][96] of the Gram (window rows, row table,
@@ -307,6 +310,7 @@ struct letkf_solve_args {
   int k;
   int64_t ncol;
   const double* C;
+  int c32;  // C holds FP32 entries (letkf_gram_args::c32)
   const double* b;
   const int* nloc;
   double rho;
