@@ -85,8 +85,10 @@ __device__ inline double wave_total(double v) {
 //   A' <- A' - u w^T - w u^T   (A' = A[j+1:, j+1:]),  e_j = alpha.
 // Lane r keeps row r of A in registers (static column index); lane r's
 // entry of column j is picked from its registers in the step before.  The
-// reflector j is stored as a full row U[j][r] = u at r > j, 0 at 0 < r <= j,
-// and tau_j in U[j][0] (never a reflector position).
+// reflector j is stored in row U[j] from position b = 16 floor(j / 16) on:
+// tau_j at r = b (never a reflector position), 0 at b < r <= j, u at r > j;
+// positions r < b are not written (E3 reads each 16-row chunk of reflectors
+// from the chunk's first row index on).
 // ---------------------------------------------------------------------------
 constexpr int E1_CH = 2;  // step chunk of k_trd (static column ranges start at the chunk)
 
@@ -142,7 +144,10 @@ void k_trd(int64_t ncol, const double* __restrict__ C, int c32,
         const double ui = nz ? xi - (lane == j + 1 ? alpha : 0.0) : 0.0;
         const double tau = nz ? 2.0 / (nrm2 - 2.0 * alpha * x0 + alpha * alpha) : 0.0;
         if (lane == 0) { dg[j] = readlane_f64(xc, j); dg[n + j] = alpha; }
-        if (lane < n) U[(size_t)j * n + lane] = lane == 0 ? tau : ui;
+        {
+          const int rb0 = 16 * (j / 16);
+          if (lane < n && lane >= rb0) U[(size_t)j * n + lane] = lane == rb0 ? tau : ui;
+        }
         // x = 0: tau = u = 0, the step below leaves A unchanged (H = I)
         ub[lane] = ui;
         wave_lds_sync();
@@ -399,6 +404,19 @@ __device__ inline double fast_div(double a, double b) {
 
 constexpr int E3_SUB = 4;  // reflectors per static range in the back-transform
 
+// global_load_lds instructions that stage chunk c of the Householder rows in
+// E3, 64 dwords each: chunk 0 (rows 0 .. min(15, N - 3)) as one contiguous
+// block, chunk c > 0 (rows 16c .. min(16c + 15, N - 3)) row by row from
+// position 16c (0 for a chunk past the last)
+template <int N>
+constexpr int e3_stage_ops(int c) {
+  const int jhi = 16 * c + 15 < N - 3 ? 16 * c + 15 : N - 3;
+  const int rows = jhi - 16 * c + 1;
+  if (rows <= 0) return 0;
+  if (c == 0) return (rows * N * 8 + 255) / 256;
+  return rows * ((2 * (N - 16 * c) + 63) / 64);
+}
+
 // LDS of one column in E3, in doubles: Z, then sd, se, sl, sy, sb, swt (64
 // each) and the near-pair dot products [64][E3_BAND].
 constexpr int E3_BAND = 8;  // near pairs (j, j+1..j+BAND) of the sorted spectrum
@@ -608,29 +626,48 @@ __device__ __forceinline__ bool e3_column(
     return false;
   }
   // ---- z <- Q z with Q = H_0 ... H_{n-3} ----
-  // The Householder rows (U[j][r] = u at r > j, 0 at 0 < r <= j, tau_j at
-  // r = 0) are the same for every lane.  They are copied into the Z area
-  // (free between the near-pair step and U_s) by global_load_lds, 256
-  // bytes per instruction and no registers held, two chunks of 16 rows
-  // ahead of their use, and read back as LDS broadcasts.  Reflectors are
-  // applied in sub-chunks of E3_SUB so the static inner range starts at the
-  // sub-chunk's first non-zero position.
+  // The Householder rows (row j from position b = 16 floor(j / 16): tau_j at
+  // b, 0 at b < r <= j, u at r > j) are the same for every lane.  They are
+  // copied into the Z area (free between the near-pair step and U_s) by
+  // global_load_lds, no registers held, two chunks of 16 rows ahead of their
+  // use, and read back as LDS broadcasts; chunk c's rows from position 16c
+  // on only (the entries before are zero for every reflector of the chunk),
+  // one row at a time.  Reflectors are applied in sub-chunks of E3_SUB so
+  // the static inner range starts at the sub-chunk's first non-zero position.
   {
     const double* Ucol = Ug + (size_t)g * n * n;
     constexpr int NC = (N - 2 + 15) / 16;
-    // rows 16c .. min(16c + 15, N - 3) to Z + 16c N; the last instruction
-    // of a chunk may read up to 252 bytes past it, inside the column's
-    // n x n block (and inside Z)
+    // rows 16c .. min(16c + 15, N - 3), positions [16c, N), to the same
+    // places of Z (row stride N): per row ceil((N - 16c) / 32) instructions
+    // of 64 dwords, all lanes (the LDS destination comes from one lane, so
+    // the copies stay outside divergent code).  A row's last instruction may
+    // run on into the next rows' first entries, which it copies to the same
+    // places: entries below the next row's chunk start (never read), or the
+    // row's own values; it stays inside the column's n x n block and Z.
+    // Chunk 0 (whole rows) goes as one contiguous block of 256-byte
+    // instructions, the last of which may run on up to 252 bytes, inside the
+    // column's n x n block.
     auto stage = [&](int c) {
       const int jlo = 16 * c;
       const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
-      const int bytes = (jhi - jlo + 1) * N * 8;
-      const char* src = (const char*)(Ucol + (size_t)jlo * N) + 4 * lane;
-      char* dst = (char*)(Z + jlo * N);
-      for (int off = 0; off < bytes; off += 256)
-        __builtin_amdgcn_global_load_lds((const void*)(src + off),
-                                         (__attribute__((address_space(3))) void*)(dst + off),
-                                         4, 0, 0);
+      if (c == 0) {
+        const char* src = (const char*)Ucol + 4 * lane;
+        char* dst = (char*)Z;
+        for (int off = 0; off < (jhi + 1) * N * 8; off += 256)
+          __builtin_amdgcn_global_load_lds((const void*)(src + off),
+                                           (__attribute__((address_space(3))) void*)(dst + off),
+                                           4, 0, 0);
+        return;
+      }
+      const int dw = 2 * (N - jlo);  // dwords per row
+      for (int j = jlo; j <= jhi; ++j) {
+        const char* src = (const char*)(Ucol + (size_t)j * N + jlo) + 4 * lane;
+        char* dst = (char*)(Z + j * N + jlo);
+        for (int d0 = 0; d0 < dw; d0 += 64)
+          __builtin_amdgcn_global_load_lds((const void*)(src + 4 * d0),
+                                           (__attribute__((address_space(3))) void*)(dst + 4 * d0),
+                                           4, 0, 0);
+      }
     };
     stage(NC - 1);
     if (NC > 1) stage(NC - 2);
@@ -638,9 +675,10 @@ __device__ __forceinline__ bool e3_column(
     for (int c = NC - 1; c >= 0; --c) {
       const int jlo = 16 * c;
       const int jhi = (16 * c + 15 < N - 3) ? 16 * c + 15 : N - 3;
-      // chunk c has landed once at most chunk c-1's copies (a full chunk:
-      // 16 N 8 / 256 = N / 2 instructions) are outstanding
-      if (c > 0) wait_vmcnt<N / 2>();
+      // chunk c has landed once at most chunk c-1's copies are outstanding
+      if (c == 3) wait_vmcnt<e3_stage_ops<N>(2)>();
+      else if (c == 2) wait_vmcnt<e3_stage_ops<N>(1)>();
+      else if (c == 1) wait_vmcnt<e3_stage_ops<N>(0)>();
       else wait_vmcnt<0>();
       if (c >= 2) stage(c - 2);
       // sub-chunks of E3_SUB reflectors, each with its own static range
@@ -652,7 +690,7 @@ __device__ __forceinline__ bool e3_column(
 #pragma unroll 1
         for (int j = shi; j >= slo; --j) {
           const double* u = Z + j * N;
-          const double tau = u[0];
+          const double tau = u[jlo];
           double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
           for (int i = slo + 1; i < N; ++i) {
