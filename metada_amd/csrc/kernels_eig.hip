@@ -632,8 +632,10 @@ __device__ __forceinline__ bool e3_column(
   // global_load_lds, no registers held, two chunks of 16 rows ahead of their
   // use, and read back as LDS broadcasts; chunk c's rows from position 16c
   // on only (the entries before are zero for every reflector of the chunk),
-  // one row at a time.  Reflectors are applied in sub-chunks of E3_SUB so
-  // the static inner range starts at the sub-chunk's first non-zero position.
+  // one row at a time.  (The compiler makes the first LDS read of a chunk
+  // wait for every copy still outstanding, so the copies issued before it
+  // land first.)  Reflectors are applied in sub-chunks of E3_SUB so the
+  // static inner range starts at the sub-chunk's first non-zero position.
   {
     const double* Ucol = Ug + (size_t)g * n * n;
     constexpr int NC = (N - 2 + 15) / 16;
