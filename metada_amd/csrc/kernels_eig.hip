@@ -250,6 +250,24 @@ __device__ inline void hyp_rinv(double f, double g, double& r, double& inv) {
 
 __device__ inline bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 
+// Smallest / largest v in [0, 64) over the lanes where p holds, wave-uniform,
+// by bisection on ballots (6 steps, no cross-lane data movement); 63 / 0
+// when p holds nowhere.
+__device__ inline int wave_min_idx(int v, bool p) {
+  int t = 0;
+#pragma unroll
+  for (int b = 32; b >= 1; b >>= 1)
+    if (!wave_any(p && v < t + b)) t += b;
+  return t;
+}
+__device__ inline int wave_max_idx(int v, bool p) {
+  int t = 0;
+#pragma unroll
+  for (int b = 32; b >= 1; b >>= 1)
+    if (wave_any(p && v >= t + b)) t += b;
+  return t;
+}
+
 template <int N>
 __device__ inline void cmp_swap(double (&d)[N], int i, int j) {
   const double a = d[i], b = d[j];
@@ -301,12 +319,16 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
   // inactive lanes hold zeros: every e is negligible and l runs to n-1 at once
   int l = 0, iter = 0;
   bool fail = false, done = false;
+  // smallest l and largest m over the lanes still iterating: the loops below
+  // skip indices outside [lo, hi] by a scalar branch (per-lane conditions
+  // still mask the work inside)
+  int lo = 0;
   while (true) {
     // negligible off-diagonals e_i (bit i), restricted to i >= the smallest l
     uint64_t small = 0;
 #pragma unroll
     for (int i = 0; i < n - 1; ++i) {
-      if (!wave_any(!done && l <= i)) continue;
+      if (i < lo) continue;
       const bool z = fabs(e[i]) <= eps * (fabs(d[i]) + fabs(d[i + 1]));
       small |= z ? (1ull << i) : 0ull;
     }
@@ -321,10 +343,12 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
     const int m = ms ? __builtin_ctzll(ms) : n - 1;
     if (!done && ++iter > 60) { fail = true; done = true; }
     if (!wave_any(!done)) break;
+    lo = wave_min_idx(l, !done);
+    const int hi = wave_max_idx(m, !done);
     double dl = 0.0, dl1 = 0.0, el = 1.0, dm = 0.0;
 #pragma unroll
     for (int i = 0; i < n; ++i) {
-      if (!wave_any(!done && l <= i && i <= m)) continue;
+      if (i < lo || i > hi) continue;
       if (i == l) {
         dl = d[i];
         el = e[i];
@@ -340,8 +364,8 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
     bool run = !done;  // cleared when this lane's sweep ends early (underflow)
 #pragma unroll
     for (int i = n - 2; i >= 0; --i) {
+      if (i < lo || i >= hi) continue;
       const bool act = run && l <= i && i < m;
-      if (!wave_any(act)) continue;
       const double f = s * e[i], b = c * e[i];
       double r, inv;
       hyp_rinv(f, gg, r, inv);
