@@ -776,8 +776,11 @@ static letkf_plan plan_letkf(mda_ctx* c, const mda_letkf_params* prm, mda_ensemb
   bg.coord = geom->coord;
   if (geom->coord == MDA_COORD_GEOGRAPHIC) {
     double ddeg = prm->radius / 6371.0 * 180.0 / 3.14159265358979323846;
-    // at least 1/16 degree: 2880 x 5760 buckets stay under the 2^24 key limit
-    double step = std::min(10.0, std::max(0.0625, ddeg / 2.0));
+    // a quarter of the radius: a window spans about 9 bucket rows (inside the
+    // Gram's 16-row range table), each trimmed to the cap, and holds about
+    // 1.3x the local observations (1.6x at half the radius); at least 1/16
+    // degree: 2880 x 5760 buckets stay under the 2^24 key limit
+    double step = std::min(10.0, std::max(0.0625, ddeg / 4.0));
     // bucket sizes divide the latitude and longitude ranges exactly
     bg.nrow = (int)std::ceil(180.0 / step);
     bg.ncolb = (int)std::ceil(360.0 / step);
