@@ -323,6 +323,12 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
   // skip indices outside [lo, hi] by a scalar branch (per-lane conditions
   // still mask the work inside)
   int lo = 0;
+  // sweeps of the wavefront: a lane runs at most 61 per value of l and l
+  // takes at most n - 1 values, so more than 61 N means a broken invariant.
+  // (The count also keeps the register allocator from spreading the
+  // selection's values over the sweep: 291 registers instead of 414 at
+  // N = 64, far fewer accumulator-register moves.)
+  int wit = 0;
   while (true) {
     // negligible off-diagonals e_i (bit i), restricted to i >= the smallest l
     uint64_t small = 0;
@@ -356,6 +362,7 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
       }
       if (i == m) dm = d[i];
     }
+    ++wit;
     // shift from the leading 2x2 of the block
     double gg = (dl1 - dl) / (2.0 * el);
     double rr = hyp2(gg, 1.0);
@@ -390,6 +397,7 @@ void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __r
       }
     }
   }
+  if (wit > 61 * N) fail = true;
   sort_network<N>(d);
   if (active) {
     double* lg = lam + (size_t)g * n;
