@@ -1017,6 +1017,9 @@ __global__ __launch_bounds__(256) void k_trd_eigvals_bisect(int64_t ncol,
     const int j = lane + 64 * q;
     a[q] = lo;
     b[q] = hi;
+    // rolled (by 4): fully unrolled, the 64 points and counts would sit in
+    // registers (216 VGPRs, two waves per SIMD); rolled the kernel needs 44
+#pragma unroll 4
     for (int m = 0; m < 64; ++m) {
       const double xm = msx[wv][m];
       if (msc[wv][m] <= j) a[q] = fmax(a[q], xm);
