@@ -301,6 +301,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1)))
 void k_trd_eigvals(int64_t ncol, const int* __restrict__ nloc, const double* __restrict__ de,
                    double* __restrict__ lam, int* __restrict__ flag) {
   constexpr int n = N;
+  // the index bisections below cover [0, 64); wider problems use k_trd_eigvals_bisect
+  static_assert(N >= 2 && N <= 64, "one lane per column holds at most 64 diagonal entries");
   const int lane = threadIdx.x;
   const int64_t g = (int64_t)blockIdx.x * 64 + lane;
   const bool active = g < ncol && nloc[g] != 0;
